@@ -1,0 +1,171 @@
+"""Benchmark of the cross-section hot path on one MI355X.
+
+A step is one full cross-section (line intensities + Voigt accumulation over
+the whole grid) of one (T, P) point, with the line list already resident in
+HBM.  The default workload is the precision form: 1e8 synthetic ExoMol-format
+lines (1e6 states), T = 1500 K, P = 0.1 atm, grid 0-12000 cm^-1 at 0.005 cm^-1
+(2.4e6 bins), 25 cm^-1 cutoff, FP32 profiles with compensated sums and FP64
+intensities/offsets.  The same workload also runs once through the plain FP64
+path for the speed ratio and the precision report (max/RMS relative error of
+the mixed grid against the FP64 grid, bins >= 1e-6 of the maximum; the FP64
+path itself is held to 1e-10 of the oracle by the parity tests).
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                                              [--workload voigt_1e8|voigt_1e7]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compensated add
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ll, wl, Q, gam, target_s=15.0):
+    """The oracle (C restatement of the reference's CPU manager, all host threads)
+    on a contiguous bin sub-range of the same workload."""
+    from oracle import pyoracle as po
+    from gpu_cross_amd.manager import make_grid
+
+    start, end, M = wl["grid"]
+    freq, d = make_grid(start, end, M)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    cores = min(cores, 16)
+    c = po.consts(po.VOIGT, wl["T"], Q, d, ll.mass, wl["P"], 500.0, cutoff=wl["cutoff"])
+    mid = M // 2
+    bins = 200
+    rate = None
+    while True:
+        j0 = mid - bins // 2
+        lo_nu, hi_nu = freq[j0] - wl["cutoff"] - 2 * d, freq[j0 + bins - 1] + wl["cutoff"] + 2 * d
+        a, b = np.searchsorted(ll.nu, [lo_nu, hi_nu])
+        sl = slice(a, b)
+        args = (ll.elow[sl], ll.nu[sl], ll.aif[sl], ll.gns[sl], np.ascontiguousarray(gam[sl]), ll.nexp[sl])
+        n_ev = po.count_evals(c, ll.nu[sl], freq, bins, j0)
+        t0 = time.perf_counter()
+        po.cross_section(c, *args, freq, bins, j0, cores)
+        dt = time.perf_counter() - t0
+        rate = n_ev / dt
+        if dt > target_s / 4 or bins >= M // 4:
+            break
+        bins = int(min(M // 4, bins * max(2.0, min(8.0, target_s / max(dt, 1e-3)))))
+    return {"value": rate, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"bins [{j0},{j0 + bins}) of the same grid with every line touching them "
+                      f"({b - a} lines, {n_ev:.3e} evals, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="voigt_1e8", choices=["voigt_1e8", "voigt_1e7"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from gpu_cross_amd import capi
+    from gpu_cross_amd.manager import Mi355xManager, make_grid
+    from gpu_cross_amd.synth import WORKLOADS, workload
+    from oracle import pyoracle as po
+
+    wl = WORKLOADS[args.workload]
+    t0 = time.time()
+    ll = workload(args.workload).sorted_by_nu()
+    log(f"generated {ll.n} lines in {time.time() - t0:.1f} s")
+    T, P = wl["T"], wl["P"]
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    start, end, M = wl["grid"]
+    freq, d = make_grid(start, end, M)
+    gam = ll.gamma_at(P)
+
+    def run(precision, steps, warmup):
+        m = Mi355xManager(capi.VOIGT, precision=precision, cutoff=wl["cutoff"])
+        m.InitializeVectors(M)
+        m.TransferFreq(freq, np.zeros(M), M)
+        m.InitializeConstants(0.0, T, Q, d, ll.mass, P, 500.0)
+        t1 = time.time()
+        m.TransferVectors(ll.n, ll.elow, ll.nu, ll.aif, ll.gns, gam, ll.nexp)
+        log(f"uploaded lines in {time.time() - t1:.1f} s (precision={precision})")
+        for _ in range(warmup):
+            m.ExecuteCrossSection(M, ll.n, 0)
+        m.synchronize()
+        acc_ms, int_ms = [], []
+        m.TransferFreq(freq, np.zeros(M), M)
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            m.ExecuteCrossSection(M, ll.n, 0)
+            t = m.last_timing()                      # events on the manager's stream
+            acc_ms.append(t.accumulate_ms)
+            int_ms.append(t.intensity_ms)
+            log(f"step: accumulate {t.accumulate_ms:.2f} ms, intensity {t.intensity_ms:.3f} ms, evals {t.evals:.4e}")
+        m.synchronize()
+        wall = time.perf_counter() - t1
+        out = np.zeros(M)
+        m.TransferResults(None, out, M)
+        evals = t.evals
+        m.close()
+        return dict(wall=wall, acc_ms=float(np.mean(acc_ms)), int_ms=float(np.mean(int_ms)), evals=evals,
+                    out=out / steps)
+
+    mixed = run(capi.PREC_MIXED, args.steps, args.warmup)
+    evals = mixed["evals"]
+    ms_step = mixed["wall"] / args.steps * 1e3
+    value = evals * args.steps / mixed["wall"]
+    res = {
+        "metric": "Voigt line-bin evals/s (mixed FP32+Kahan, FP64 intensity; 1e-5 rel err target)",
+        "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {ll.n} lines, {M} bins {start}-{end} cm^-1, T={T} K, P={P} atm, "
+                               f"cutoff {wl['cutoff']} cm^-1, one (T,P) cross-section per step",
+                   "lines": ll.n, "bins": M, "evals_per_step": evals},
+        "ms_per_xsec": ms_step,
+        "roofline": {"bound": "valu-fp32", "achieved": FLOP_PER_EVAL * evals / (mixed["acc_ms"] * 1e-3) / 1e12,
+                     "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": FLOP_PER_EVAL * evals / (mixed["acc_ms"] * 1e-3) / PEAK_FP32, "traffic": None,
+                     "kernel": "k_accum_voigt_mixed", "kernel_ms": mixed["acc_ms"],
+                     "flop_per_eval": FLOP_PER_EVAL},
+        "intensity_kernel": {"ms": mixed["int_ms"], "bytes": ll.n * 68,
+                             "GB_per_s": ll.n * 68 / (mixed["int_ms"] * 1e-3) / 1e9},
+    }
+    if not args.no_fp64:
+        f64 = run(capi.PREC_FP64, 1, 0)
+        ref = f64["out"]
+        mask = ref >= 1e-6 * ref.max()
+        rel = np.abs(mixed["out"][mask] - ref[mask]) / ref[mask]
+        res["fp64_path"] = {"evals_per_s": f64["evals"] / (f64["wall"]), "accumulate_ms": f64["acc_ms"],
+                            "frac_fp64_peak": FLOP_PER_EVAL * f64["evals"] / (f64["acc_ms"] * 1e-3) / PEAK_FP64}
+        res["speedup_vs_fp64"] = value / res["fp64_path"]["evals_per_s"]
+        res["precision"] = {"max_rel_err": float(rel.max()), "rms_rel_err": float(np.sqrt(np.mean(rel ** 2))),
+                            "bins_checked": int(mask.sum()), "against": "FP64 GPU path (oracle-parity 1e-10)"}
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(ll, wl, Q, gam)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

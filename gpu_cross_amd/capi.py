@@ -1,0 +1,111 @@
+"""ctypes binding of libgx.so's C ABI (include/gx.h).
+
+The library is built in-tree (``gpu_cross_amd/libgx.so``) by
+``make -C gpu_cross_amd/csrc``; ``load()`` never falls back to anything else:
+a missing library raises ``LibraryMissing``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgx.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "gx.h")
+
+GX_OK, GX_INVALID_ARG, GX_OUT_OF_MEMORY, GX_NO_DEVICE, GX_DEVICE_ERROR, GX_BUSY, GX_NOT_FOUND = range(7)
+DOPPLER, VOIGT = 1, 2
+CONV_OPENMP, CONV_GPU = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class GxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gx status {status}: {msg}")
+        self.status = status
+
+
+class Consts(ctypes.Structure):
+    _fields_ = [("half_width", ctypes.c_double), ("temperature", ctypes.c_double), ("partition", ctypes.c_double),
+                ("dfreq", ctypes.c_double), ("mean_mass", ctypes.c_double), ("pressure", ctypes.c_double),
+                ("ref_temp", ctypes.c_double), ("cutoff", ctypes.c_double), ("doppler_widths", ctypes.c_double),
+                ("convention", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("intensity_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float), ("evals", ctypes.c_longlong),
+                ("lines", ctypes.c_longlong)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
+    return LIB_PATH
+
+
+def declared_symbols() -> list:
+    """Every function name declared in include/gx.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(gx_\w+)\s*\(", text, flags=re.M)))
+
+
+_lib = None
+
+
+def _sig(L, name, res, args):
+    f = getattr(L, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built; run `make -C gpu_cross_amd/csrc`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    _sig(L, "gx_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(P)])
+    _sig(L, "gx_initialize_vectors", ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)])
+    _sig(L, "gx_initialize_constants", ctypes.c_int, [P, ctypes.POINTER(Consts)])
+    _sig(L, "gx_transfer_freq", ctypes.c_int, [P, _dp, _dp, ctypes.c_int])
+    _sig(L, "gx_transfer_vectors", ctypes.c_int, [P, ctypes.c_size_t, _dp, _dp, _dp, _ip, _dp, _dp])
+    _sig(L, "gx_execute", ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int])
+    _sig(L, "gx_ready", ctypes.c_int, [P])
+    _sig(L, "gx_transfer_results", ctypes.c_int, [P, _dp, _dp, ctypes.c_int])
+    _sig(L, "gx_cleanup", ctypes.c_int, [P])
+    _sig(L, "gx_destroy", ctypes.c_int, [P])
+    _sig(L, "gx_get_ntrans", ctypes.c_size_t, [P])
+    _sig(L, "gx_synchronize", ctypes.c_int, [P])
+    _sig(L, "gx_last_timing", ctypes.c_int, [P, ctypes.POINTER(Timing)])
+    _sig(L, "gx_last_error", ctypes.c_char_p, [])
+    _sig(L, "gx_version", ctypes.c_char_p, [])
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != GX_OK:
+        raise GxError(status, load().gx_last_error().decode(errors="replace"))
+
+
+def dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_ip)

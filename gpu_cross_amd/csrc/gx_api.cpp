@@ -1,0 +1,461 @@
+// gx_api.cpp -- host side of libgx.so: the single-GPU manager context behind
+// the C ABI of include/gx.h.
+//
+// One gx_ctx replaces one reference GpuManager (GPUManager.cpp:5-189) with the
+// BaseManager lifecycle (BaseManager.h:22-30):
+//   create -> initialize_vectors(M) -> transfer_freq -> initialize_constants
+//   -> {transfer_vectors(n); execute(points, n, ib)}* -> transfer_results -> cleanup.
+// Differences that are deliberate: the uploaded line list is never mutated
+// (the reference's stepone overwrites E'' and gamma in place,
+// cross_kernal.cu:780,783), per-point derived data lives in separate buffers,
+// chunks are sorted by nu on the device when the caller's order is not, and
+// errors come back as status codes instead of exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/gx.h"
+#include "gx_internal.h"
+#include "gx_common.h"
+
+namespace gx {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return fail(GX_OUT_OF_MEMORY, m);
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return fail(GX_NO_DEVICE, m);
+  return fail(GX_DEVICE_ERROR, m);
+}
+
+// ---- constants, exactly as the reference's managers derive them ----
+// defines.h:4-20 literals
+constexpr double kPlanck = 6.6260693e-27, kVellgt = 2.99792458e+10, kBoltz = 1.380658e-16;
+constexpr double kNA = 6.0221412927e23, kLn2 = 0.69314718056, kPi = 3.14159265359;
+
+PointConsts make_point_consts(int profile, const gx_consts& c, double pressure_scale) {
+  PointConsts p{};
+  p.profile = profile;
+  p.convention = c.convention;
+  p.temperature = c.temperature;
+  p.partition = c.partition;
+  p.ref_temp = c.ref_temp;
+  p.beta = kPlanck * kVellgt / (kBoltz * c.temperature);               // OpenMPManager.cpp:66
+  if (c.convention == GX_CONV_GPU) {
+    p.cmcoef = 1.0 / (8.0 * std::acos(-1.0) * 2.99792458e10);           // GPUManager.cpp:31,39
+    p.dpwcoef = std::sqrt(2.0 * kBoltz * c.temperature * kNA / (c.mean_mass)) / kVellgt;   // cross_kernal.cu:938
+  } else {
+    p.cmcoef = 1.0 / (8.0 * kPi * kVellgt);                             // OpenMPManager.cpp:65
+    p.dpwcoef = std::sqrt(2.0 * kLn2 * kBoltz * c.temperature * kNA / (c.mean_mass)) / kVellgt;  // :191
+  }
+  p.cutoff = reference_cutoff(profile, c.pressure, c.cutoff);
+  p.doppler_widths = profile == GX_DOPPLER ? c.doppler_widths : 0.0;
+  p.pressure_scale = pressure_scale;
+  return p;
+}
+
+double reference_cutoff(int profile, double pressure, double cutoff) {
+  if (cutoff > 0.0) return cutoff;
+  if (profile == GX_DOPPLER) return 100.0;     // defines.h:20, DopplerProfile.cpp:4
+  return std::min(25.0 * pressure, 100.0);     // OpenMPManager.cpp:192, VoigtProfile.cpp:49
+}
+
+double max_cutoff(const PointConsts& p, double max_nu, double dfreq) {
+  if (p.profile == GX_DOPPLER && p.doppler_widths > 0.0)
+    return p.doppler_widths * (std::max(max_nu, 1e-6) * p.dpwcoef) * (1.0 + 1e-12) + dfreq;
+  return p.cutoff;
+}
+
+// ---- device line storage shared by the manager, the table and the server ----
+int LineStore::reserve(size_t n, bool voigt) {
+  if (n <= cap && (!voigt || gam)) return GX_OK;
+  release();
+  const size_t c = std::max<size_t>(n, 1);
+  hipError_t e;
+  if ((e = hipMalloc(&nu, c * 8)) != hipSuccess) return hip_fail(e, "hipMalloc(nu)");
+  if ((e = hipMalloc(&elow, c * 8)) != hipSuccess) return hip_fail(e, "hipMalloc(elow)");
+  if ((e = hipMalloc(&aif, c * 8)) != hipSuccess) return hip_fail(e, "hipMalloc(aif)");
+  if ((e = hipMalloc(&gns, c * 4)) != hipSuccess) return hip_fail(e, "hipMalloc(gns)");
+  if (voigt) {
+    if ((e = hipMalloc(&gam, c * 8)) != hipSuccess) return hip_fail(e, "hipMalloc(gamma)");
+    if ((e = hipMalloc(&nexp, c * 8)) != hipSuccess) return hip_fail(e, "hipMalloc(nexp)");
+  }
+  cap = c;
+  return GX_OK;
+}
+
+void LineStore::release() {
+  hipFree(nu); hipFree(elow); hipFree(aif); hipFree(gns); hipFree(gam); hipFree(nexp);
+  nu = elow = aif = gam = nexp = nullptr;
+  gns = nullptr;
+  cap = 0;
+  n = 0;
+}
+
+// Upload n lines (host SoA) and leave them sorted by nu on the device.
+int LineStore::upload(size_t nlines, const double* h_e, const double* h_nu, const double* h_a, const int* h_g,
+                      const double* h_gam, const double* h_n, hipStream_t s) {
+  const bool voigt = h_gam != nullptr;
+  int rc = reserve(nlines, voigt);
+  if (rc) return rc;
+  bool sorted = true;
+  double mx = 0.0;
+  for (size_t i = 0; i < nlines; ++i) {
+    if (!(h_nu[i] == h_nu[i])) return fail(GX_INVALID_ARG, "transfer_vectors: nu is NaN");
+    if (i && h_nu[i] < h_nu[i - 1]) sorted = false;
+    mx = std::max(mx, h_nu[i]);
+  }
+  max_nu = mx;
+  n = nlines;
+  hipError_t e;
+  if (nlines == 0) return GX_OK;
+  if (sorted) {
+    if ((e = hipMemcpyAsync(nu, h_nu, nlines * 8, hipMemcpyHostToDevice, s)) ||
+        (e = hipMemcpyAsync(elow, h_e, nlines * 8, hipMemcpyHostToDevice, s)) ||
+        (e = hipMemcpyAsync(aif, h_a, nlines * 8, hipMemcpyHostToDevice, s)) ||
+        (e = hipMemcpyAsync(gns, h_g, nlines * 4, hipMemcpyHostToDevice, s)))
+      return hip_fail(e, "transfer_vectors: copy");
+    if (voigt && ((e = hipMemcpyAsync(gam, h_gam, nlines * 8, hipMemcpyHostToDevice, s)) ||
+                  (e = hipMemcpyAsync(nexp, h_n, nlines * 8, hipMemcpyHostToDevice, s))))
+      return hip_fail(e, "transfer_vectors: copy");
+    if ((e = hipStreamSynchronize(s))) return hip_fail(e, "transfer_vectors: sync");
+    return GX_OK;
+  }
+  // Unsorted chunk: stage raw arrays, radix-sort (nu, index) pairs, gather the rest.
+  const int ni = (int)nlines;
+  double *t_nu = nullptr, *t_e = nullptr, *t_a = nullptr, *t_gam = nullptr, *t_n = nullptr;
+  int* t_g = nullptr;
+  uint32_t *idx_in = nullptr, *idx_out = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  auto cleanup = [&]() {
+    hipFree(t_nu); hipFree(t_e); hipFree(t_a); hipFree(t_g); hipFree(t_gam); hipFree(t_n);
+    hipFree(idx_in); hipFree(idx_out); hipFree(tmp);
+  };
+  auto bail = [&](hipError_t err, const char* what) { cleanup(); return hip_fail(err, what); };
+  if ((e = hipMalloc(&t_nu, nlines * 8)) || (e = hipMalloc(&t_e, nlines * 8)) || (e = hipMalloc(&t_a, nlines * 8)) ||
+      (e = hipMalloc(&t_g, nlines * 4)) || (e = hipMalloc(&idx_in, nlines * 4)) || (e = hipMalloc(&idx_out, nlines * 4)))
+    return bail(e, "transfer_vectors: staging alloc");
+  if (voigt && ((e = hipMalloc(&t_gam, nlines * 8)) || (e = hipMalloc(&t_n, nlines * 8))))
+    return bail(e, "transfer_vectors: staging alloc");
+  if ((e = hipMemcpyAsync(t_nu, h_nu, nlines * 8, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(t_e, h_e, nlines * 8, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(t_a, h_a, nlines * 8, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(t_g, h_g, nlines * 4, hipMemcpyHostToDevice, s)))
+    return bail(e, "transfer_vectors: copy");
+  if (voigt && ((e = hipMemcpyAsync(t_gam, h_gam, nlines * 8, hipMemcpyHostToDevice, s)) ||
+                (e = hipMemcpyAsync(t_n, h_n, nlines * 8, hipMemcpyHostToDevice, s))))
+    return bail(e, "transfer_vectors: copy");
+  if ((e = launch_iota(idx_in, ni, s))) return bail(e, "transfer_vectors: iota");
+  if ((e = sort_pairs_by_nu(nullptr, tmp_bytes, t_nu, nu, idx_in, idx_out, ni, s)))
+    return bail(e, "transfer_vectors: sort size");
+  if ((e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)))) return bail(e, "transfer_vectors: sort alloc");
+  if ((e = sort_pairs_by_nu(tmp, tmp_bytes, t_nu, nu, idx_in, idx_out, ni, s))) return bail(e, "transfer_vectors: sort");
+  if ((e = launch_gather_lines(idx_out, t_e, t_a, t_g, t_gam, t_n, elow, aif, gns, gam, nexp, ni, s)))
+    return bail(e, "transfer_vectors: gather");
+  if ((e = hipStreamSynchronize(s))) return bail(e, "transfer_vectors: sync");
+  cleanup();
+  return GX_OK;
+}
+
+int Derived::reserve(size_t n) {
+  if (n <= cap) return GX_OK;
+  release();
+  const size_t c = std::max<size_t>(n, 1);
+  hipError_t e;
+  if ((e = hipMalloc(&d.S, c * 8)) || (e = hipMalloc(&d.g, c * 8)) || (e = hipMalloc(&d.y, c * 8)) ||
+      (e = hipMalloc(&d.ib, c * 4)) || (e = hipMalloc(&d.ie, c * 4)))
+    return hip_fail(e, "hipMalloc(derived)");
+  cap = c;
+  return GX_OK;
+}
+
+void Derived::release() {
+  hipFree(d.S); hipFree(d.g); hipFree(d.y); hipFree(d.ib); hipFree(d.ie);
+  d = DerivedF64{};
+  cap = 0;
+}
+
+int Ranges::reserve(int nb) {
+  if (nb <= cap) return GX_OK;
+  hipFree(r);
+  r = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(&r, sizeof(int2) * (size_t)std::max(nb, 1));
+  if (e) return hip_fail(e, "hipMalloc(ranges)");
+  cap = std::max(nb, 1);
+  return GX_OK;
+}
+
+void Ranges::release() {
+  hipFree(r);
+  r = nullptr;
+  cap = 0;
+}
+
+// One (T,P) point over [start_idx, start_idx+points) of a grid: prep, ranges, accumulate.
+int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, int nlines, int precision,
+              Derived& der, Ranges& rng, double* intens, unsigned long long* d_evals, hipStream_t s,
+              hipEvent_t* ev) {
+  LinesView lv{ls.nu, ls.elow, ls.aif, ls.gns, ls.gam, ls.nexp, nlines};
+  int rc = der.reserve((size_t)std::max(nlines, 1));
+  if (rc) return rc;
+  const bool mixed = precision == GX_PREC_MIXED && pc.profile == GX_VOIGT;
+  const int bpb = mixed ? mixed_bins_per_block() : kBinsF64;
+  const int nblocks = gv.points > 0 ? (gv.points + bpb - 1) / bpb : 0;
+  if ((rc = rng.reserve(nblocks))) return rc;
+  hipError_t e;
+  if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");
+  if ((e = launch_line_prep_f64(pc, gv, lv, der.d, d_evals, s))) return hip_fail(e, "line prep launch");
+  if (ev && (e = hipEventRecord(ev[1], s))) return hip_fail(e, "event");
+  const double cut = max_cutoff(pc, ls.max_nu, gv.dfreq);
+  if ((e = launch_block_ranges(gv, ls.nu, nlines, cut, bpb, nblocks, rng.r, s))) return hip_fail(e, "ranges launch");
+  if (ev && (e = hipEventRecord(ev[2], s))) return hip_fail(e, "event");
+  if (mixed)
+    e = launch_accum_mixed(pc, gv, ls.nu, der.d, rng.r, nblocks, intens, s);
+  else
+    e = launch_accum_f64(pc, gv, ls.nu, der.d, rng.r, nblocks, intens, s);
+  if (e) return hip_fail(e, "accumulate launch");
+  if (ev && (e = hipEventRecord(ev[3], s))) return hip_fail(e, "event");
+  return GX_OK;
+}
+
+}  // namespace gx
+
+using namespace gx;
+
+struct gx_ctx {
+  int device = 0;
+  int profile = GX_VOIGT;
+  size_t budget = 0;
+  size_t ntrans = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool timed = false;
+  int M = 0;
+  double* d_freq = nullptr;
+  double* d_intens = nullptr;
+  double start_nu = 0.0;
+  gx_consts consts{};
+  bool have_consts = false;
+  LineStore lines;
+  Derived derived;
+  Ranges ranges;
+  unsigned long long* d_evals = nullptr;
+  int last_lines = 0;
+};
+
+namespace {
+
+constexpr size_t kBytesPerLine = 44 + 32 + 52;   // resident + derived + sort staging
+constexpr size_t kMaxNtrans = size_t(1) << 30;   // keeps Nener within the int of ExecuteCrossSection
+
+int set_device(gx_ctx* c) {
+  hipError_t e = hipSetDevice(c->device);
+  return e ? hip_fail(e, "hipSetDevice") : GX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_create(int device, int profile, size_t mem_budget, gx_ctx** out) {
+  if (!out) return fail(GX_INVALID_ARG, "gx_create: out is NULL");
+  *out = nullptr;
+  if (profile != GX_DOPPLER && profile != GX_VOIGT)
+    return fail(GX_INVALID_ARG, "gx_create: only DOPPLER (1) and VOIGT (2) profiles are implemented");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e || ndev <= 0) return fail(GX_NO_DEVICE, "gx_create: no HIP device");
+  if (device < 0 || device >= ndev) return fail(GX_NO_DEVICE, "gx_create: device index out of range");
+  if ((e = hipSetDevice(device))) return hip_fail(e, "hipSetDevice");
+  size_t free_b = 0, total_b = 0;
+  if ((e = hipMemGetInfo(&free_b, &total_b))) return hip_fail(e, "hipMemGetInfo");
+  gx_ctx* c = new gx_ctx();
+  c->device = device;
+  c->profile = profile;
+  // GPUManager.cpp:12 keeps 300 MB back; 1 GiB here leaves room for scratch.
+  c->budget = mem_budget ? mem_budget : (total_b > (size_t(1) << 30) ? total_b - (size_t(1) << 30) : total_b / 2);
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) { delete c; return hip_fail(e, "stream"); }
+  for (auto& ev : c->ev)
+    if ((e = hipEventCreate(&ev))) { gx_destroy(c); return hip_fail(e, "event"); }
+  if ((e = hipMalloc(&c->d_evals, sizeof(unsigned long long)))) { gx_destroy(c); return hip_fail(e, "hipMalloc"); }
+  *out = c;
+  return GX_OK;
+}
+
+int gx_initialize_vectors(gx_ctx* c, int npoints, size_t* ntrans_out) {
+  if (!c || npoints <= 0) return fail(GX_INVALID_ARG, "initialize_vectors: bad arguments");
+  int rc = set_device(c);
+  if (rc) return rc;
+  hipFree(c->d_freq);
+  hipFree(c->d_intens);
+  c->d_freq = c->d_intens = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&c->d_freq, sizeof(double) * (size_t)npoints)) ||
+      (e = hipMalloc(&c->d_intens, sizeof(double) * (size_t)npoints)))
+    return hip_fail(e, "initialize_vectors: hipMalloc(grid)");
+  if ((e = hipMemset(c->d_intens, 0, sizeof(double) * (size_t)npoints))) return hip_fail(e, "memset");
+  c->M = npoints;
+  const size_t grid_bytes = 16 * (size_t)npoints;
+  const size_t avail = c->budget > grid_bytes ? c->budget - grid_bytes : 0;
+  c->ntrans = std::min(avail / kBytesPerLine, kMaxNtrans);   // GPUManager.cpp:55-58
+  if (ntrans_out) *ntrans_out = c->ntrans;
+  return GX_OK;
+}
+
+int gx_initialize_constants(gx_ctx* c, const gx_consts* k) {
+  if (!c || !k) return fail(GX_INVALID_ARG, "initialize_constants: NULL argument");
+  if (!(k->temperature > 0) || !(k->mean_mass > 0) || !(k->dfreq > 0) || !(k->partition > 0) ||
+      !(k->ref_temp > 0) || !std::isfinite(k->pressure))
+    return fail(GX_INVALID_ARG, "initialize_constants: temperature, mean_mass, dfreq, partition and ref_temp must be > 0");
+  if (k->convention != GX_CONV_OPENMP && k->convention != GX_CONV_GPU)
+    return fail(GX_INVALID_ARG, "initialize_constants: unknown convention");
+  if (k->precision != GX_PREC_FP64 && k->precision != GX_PREC_MIXED)
+    return fail(GX_INVALID_ARG, "initialize_constants: unknown precision");
+  c->consts = *k;
+  c->have_consts = true;
+  return GX_OK;
+}
+
+int gx_transfer_freq(gx_ctx* c, const double* freq, const double* intens, int n) {
+  if (!c || !freq || n <= 0) return fail(GX_INVALID_ARG, "transfer_freq: bad arguments");
+  if (n != c->M) return fail(GX_INVALID_ARG, "transfer_freq: N differs from initialize_vectors");
+  int rc = set_device(c);
+  if (rc) return rc;
+  hipError_t e;
+  if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e, "sync");
+  if ((e = hipMemcpy(c->d_freq, freq, sizeof(double) * (size_t)n, hipMemcpyHostToDevice)))
+    return hip_fail(e, "transfer_freq: copy");
+  if (intens)
+    e = hipMemcpy(c->d_intens, intens, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
+  else
+    e = hipMemset(c->d_intens, 0, sizeof(double) * (size_t)n);
+  if (e) return hip_fail(e, "transfer_freq: intens");
+  c->start_nu = freq[0];   // OpenMPManager.cpp:87
+  return GX_OK;
+}
+
+int gx_transfer_vectors(gx_ctx* c, size_t nener, const double* energies, const double* nu, const double* aif,
+                        const int* gns, const double* gamma, const double* nexp) {
+  if (!c) return fail(GX_INVALID_ARG, "transfer_vectors: NULL context");
+  if (nener > 0 && (!energies || !nu || !aif || !gns))
+    return fail(GX_INVALID_ARG, "transfer_vectors: NULL line array");
+  if (c->profile == GX_VOIGT && nener > 0 && (!gamma || !nexp))
+    return fail(GX_INVALID_ARG, "transfer_vectors: Voigt needs gamma and n");
+  if (nener > c->ntrans && c->ntrans) return fail(GX_INVALID_ARG, "transfer_vectors: more lines than GetNtrans()");
+  if (nener > (size_t)INT32_MAX) return fail(GX_INVALID_ARG, "transfer_vectors: chunk too large");
+  int rc = set_device(c);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(c->stream);   // GPUManager.cpp:89: wait for the previous chunk
+  if (e) return hip_fail(e, "transfer_vectors: sync");
+  const bool voigt = c->profile == GX_VOIGT;
+  return c->lines.upload(nener, energies, nu, aif, gns, voigt ? gamma : nullptr, voigt ? nexp : nullptr, c->stream);
+}
+
+int gx_execute(gx_ctx* c, int points, int n_ener, int start_idx) {
+  if (!c) return fail(GX_INVALID_ARG, "execute: NULL context");
+  if (!c->have_consts) return fail(GX_INVALID_ARG, "execute: initialize_constants not called");
+  if (!c->d_freq) return fail(GX_INVALID_ARG, "execute: initialize_vectors not called");
+  if (points < 0 || start_idx < 0 || (long long)start_idx + points > c->M)
+    return fail(GX_INVALID_ARG, "execute: bin range outside the grid");
+  if (n_ener < 0 || (size_t)n_ener > c->lines.n) return fail(GX_INVALID_ARG, "execute: N_ener exceeds the uploaded chunk");
+  int rc = set_device(c);
+  if (rc) return rc;
+  const gx_consts& k = c->consts;
+  PointConsts pc = make_point_consts(c->profile, k, 1.0);
+  GridView gv{c->d_freq, c->M, start_idx, points, c->start_nu, k.dfreq};
+  hipError_t e = hipMemsetAsync(c->d_evals, 0, sizeof(unsigned long long), c->stream);
+  if (e) return hip_fail(e, "execute: memset");
+  c->last_lines = n_ener;
+  c->timed = true;
+  return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->d_intens, c->d_evals, c->stream,
+                   c->ev);
+}
+
+int gx_ready(gx_ctx* c) {
+  if (!c) return -fail(GX_INVALID_ARG, "ready: NULL context");
+  hipError_t e = hipStreamQuery(c->stream);    // GPUManager.cpp:185-188
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return -hip_fail(e, "ready");
+}
+
+int gx_transfer_results(gx_ctx* c, double* freq, double* intens, int n) {
+  if (!c || !intens || n <= 0) return fail(GX_INVALID_ARG, "transfer_results: bad arguments");
+  if (n > c->M) return fail(GX_INVALID_ARG, "transfer_results: N exceeds the grid");
+  int rc = set_device(c);
+  if (rc) return rc;
+  hipError_t e;
+  if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e, "transfer_results: kernel");
+  if (freq && (e = hipMemcpy(freq, c->d_freq, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost)))
+    return hip_fail(e, "transfer_results: copy freq");
+  if ((e = hipMemcpy(intens, c->d_intens, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost)))
+    return hip_fail(e, "transfer_results: copy intens");
+  return GX_OK;
+}
+
+int gx_cleanup(gx_ctx* c) {
+  if (!c) return fail(GX_INVALID_ARG, "cleanup: NULL context");
+  set_device(c);
+  hipStreamSynchronize(c->stream);
+  hipFree(c->d_freq);
+  hipFree(c->d_intens);
+  c->d_freq = c->d_intens = nullptr;
+  c->M = 0;
+  c->lines.release();
+  c->derived.release();
+  c->ranges.release();
+  return GX_OK;
+}
+
+int gx_destroy(gx_ctx* c) {
+  if (!c) return GX_OK;
+  gx_cleanup(c);
+  for (auto& ev : c->ev) if (ev) hipEventDestroy(ev);
+  if (c->stream) hipStreamDestroy(c->stream);
+  hipFree(c->d_evals);
+  delete c;
+  return GX_OK;
+}
+
+size_t gx_get_ntrans(gx_ctx* c) { return c ? c->ntrans : 0; }
+
+int gx_synchronize(gx_ctx* c) {
+  if (!c) return fail(GX_INVALID_ARG, "synchronize: NULL context");
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e ? hip_fail(e, "synchronize") : GX_OK;
+}
+
+int gx_last_timing(gx_ctx* c, gx_timing* out) {
+  if (!c || !out) return fail(GX_INVALID_ARG, "last_timing: NULL argument");
+  if (!c->timed) return fail(GX_NOT_FOUND, "last_timing: nothing executed yet");
+  hipError_t e;
+  if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e, "last_timing: sync");
+  float a = 0, b = 0;
+  if ((e = hipEventElapsedTime(&a, c->ev[0], c->ev[1])) || (e = hipEventElapsedTime(&b, c->ev[2], c->ev[3])))
+    return hip_fail(e, "last_timing: events");
+  unsigned long long ev = 0;
+  if ((e = hipMemcpy(&ev, c->d_evals, sizeof(ev), hipMemcpyDeviceToHost))) return hip_fail(e, "last_timing: copy");
+  out->intensity_ms = a;
+  out->accumulate_ms = b;
+  out->evals = (long long)ev;
+  out->lines = c->last_lines;
+  return GX_OK;
+}
+
+const char* gx_last_error(void) { return g_last_error.c_str(); }
+
+const char* gx_version(void) { return "gx 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// gx_w4.h -- device evaluation of Humlicek's W4 approximation to Re w(x+iy)
+// (J. Humlicek, JQSRT 27 (1982) 437), the reference's Voigt function
+// (profiles.cpp:9-42, cross_kernal.cu:149-181).
+//
+// The region tests (|x|+y >= 15, >= 5.5, y >= 0.195|x| - 0.176) and the
+// published coefficients are the algorithm itself and are kept as they are.
+// The arithmetic is written for CDNA: only the real part is formed, region I
+// uses its closed real form
+//     Re = c y (0.5 + x^2 + y^2) / ((0.5 + y^2 - x^2)^2 + 4 x^2 y^2),
+// and regions II-IV use explicit real/imaginary pairs with a single real
+// division for the final quotient.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gx {
+
+struct cplx {
+  double re, im;
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cplx cadd(double r, cplx a) { return {r + a.re, a.im}; }
+__device__ __forceinline__ cplx csub(double r, cplx a) { return {r - a.re, -a.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ double cdiv_re(cplx a, cplx b) {
+  return (a.re * b.re + a.im * b.im) / (b.re * b.re + b.im * b.im);
+}
+
+// Region I (|x| + y >= 15) in closed real form, FP64.
+__device__ __forceinline__ double w4_region1(double x, double y) {
+  const double x2 = x * x, y2 = y * y;
+  const double dr = 0.5 + y2 - x2;
+  return 0.5641896 * y * (0.5 + x2 + y2) / (dr * dr + 4.0 * x2 * y2);
+}
+
+// Regions II-IV, FP64 (x >= 0 assumed by the callers; only |x| enters).
+__device__ __noinline__ double w4_inner(double x, double y) {
+  const cplx t{y, -x};
+  const double s = x + y;
+  if (s >= 5.5) {  // region II
+    const cplx u = cmul(t, t);
+    const cplx num = cmul(t, cadd(1.410474, cscale(u, 0.5641896)));
+    const cplx den = cadd(0.75, cmul(u, cadd(3.0, u)));
+    return cdiv_re(num, den);
+  }
+  if (y >= 0.195 * x - 0.176) {  // region III
+    cplx n = cscale(t, 0.5642236);
+    n = cmul(t, cadd(3.778987, n));
+    n = cmul(t, cadd(11.96482, n));
+    n = cmul(t, cadd(20.20933, n));
+    n = cadd(16.4955, n);
+    cplx d = cadd(6.699398, t);
+    d = cadd(21.69274, cmul(t, d));
+    d = cadd(39.27121, cmul(t, d));
+    d = cadd(38.82363, cmul(t, d));
+    d = cadd(16.4955, cmul(t, d));
+    return cdiv_re(n, d);
+  }
+  // region IV: exp(u) - t P(u) / Q(u)
+  const cplx u = cmul(t, t);
+  cplx p = cscale(u, 0.56419);
+  p = csub(1.320522, p);
+  p = csub(35.76683, cmul(u, p));
+  p = csub(219.0313, cmul(u, p));
+  p = csub(1540.787, cmul(u, p));
+  p = csub(3321.9905, cmul(u, p));
+  p = csub(36183.31, cmul(u, p));
+  cplx q = csub(1.841439, u);
+  q = csub(61.57037, cmul(u, q));
+  q = csub(364.2191, cmul(u, q));
+  q = csub(2186.181, cmul(u, q));
+  q = csub(9022.228, cmul(u, q));
+  q = csub(24322.84, cmul(u, q));
+  q = csub(32066.6, cmul(u, q));
+  double sn, cs;
+  sincos(u.im, &sn, &cs);
+  return exp(u.re) * cs - cdiv_re(cmul(t, p), q);
+}
+
+// Full W4, FP64.  x >= 0.
+__device__ __forceinline__ double w4(double x, double y) {
+  if (x + y >= 15.0) return w4_region1(x, y);
+  return w4_inner(x, y);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 forms used by the mixed-precision kernels.
+// ---------------------------------------------------------------------------
+
+// Regions II-III in FP32 and region IV in FP64 (its exp(u) - tP/Q difference
+// cancels by up to two digits near the Doppler core).
+__device__ __noinline__ float w4_inner_f32(float x, float y) {
+  const float s = x + y;
+  if (s >= 5.5f) {  // region II
+    const float ur = y * y - x * x, ui = -2.0f * x * y;
+    // num = t (1.410474 + 0.5641896 u)
+    const float ar = 1.410474f + 0.5641896f * ur, ai = 0.5641896f * ui;
+    const float nr = y * ar + x * ai, ni = y * ai - x * ar;
+    // den = 0.75 + u (3 + u)
+    const float br = 3.0f + ur, bi = ui;
+    const float dr = 0.75f + (ur * br - ui * bi), di = ur * bi + ui * br;
+    return (nr * dr + ni * di) / (dr * dr + di * di);
+  }
+  if (y >= 0.195f * x - 0.176f) {  // region III
+    const float tr = y, ti = -x;
+    float nr = tr * 0.5642236f, ni = ti * 0.5642236f;
+    float ar, ai;
+    ar = 3.778987f + nr; ai = ni;  nr = tr * ar - ti * ai; ni = tr * ai + ti * ar;
+    ar = 11.96482f + nr; ai = ni;  nr = tr * ar - ti * ai; ni = tr * ai + ti * ar;
+    ar = 20.20933f + nr; ai = ni;  nr = tr * ar - ti * ai; ni = tr * ai + ti * ar;
+    nr = 16.4955f + nr;
+    float dr = 6.699398f + tr, di = ti;
+    float er, ei;
+    er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 21.69274f + er; di = ei;
+    er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 39.27121f + er; di = ei;
+    er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 38.82363f + er; di = ei;
+    er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 16.4955f + er; di = ei;
+    return (nr * dr + ni * di) / (dr * dr + di * di);
+  }
+  return (float)w4_inner((double)x, (double)y);
+}
+
+}  // namespace gx

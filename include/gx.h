@@ -1,0 +1,130 @@
+/*
+ * gx.h -- C ABI of libgx.so, the MI355X (gfx950) cross-section engine.
+ *
+ * The drop-in boundary is the reference's compute-backend interface,
+ * BaseManager (BaseManager.h:6-33), driven by BaseProfile
+ * (BaseProfile.cpp:49-74) and the profile drivers (VoigtProfile.cpp:20-26,
+ * DopplerProfile.cpp:19-25).  Each gx_* manager entry point below replaces
+ * one BaseManager method; gpu_cross_amd/csrc/mi355x_manager.h wraps them in a
+ * C++ class with the BaseManager signatures, and gpu_cross_amd/manager.py
+ * binds them for Python.
+ *
+ * Conventions: plain pointers and sizes only; every call returns a gx_status
+ * (GX_OK == 0) and leaves a message for gx_last_error() on failure, never
+ * exit()s.  Host pointers are borrowed for the duration of the call only.
+ * One context drives one GPU from one host thread.
+ */
+#ifndef GX_H
+#define GX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gx_status {
+    GX_OK = 0,
+    GX_INVALID_ARG = 1,
+    GX_OUT_OF_MEMORY = 2,
+    GX_NO_DEVICE = 3,
+    GX_DEVICE_ERROR = 4,
+    GX_BUSY = 5,
+    GX_NOT_FOUND = 6
+} gx_status;
+
+/* Input.h:9-15 ProfileType numbering (GAUSSIAN=0 and PSUEDO_VOIGT=3 are not served) */
+enum { GX_DOPPLER = 1, GX_VOIGT = 2 };
+/* Width / window / constant convention: the CPU manager's (OpenMPManager.cpp:146,191,205-206)
+ * or the CUDA manager's (cross_kernal.cu:667,938,954-957, GPUManager.cpp:31) */
+enum { GX_CONV_OPENMP = 0, GX_CONV_GPU = 1 };
+/* FP64 throughout, or FP32 profiles with compensated per-bin sums and FP64
+ * intensities / centre offsets */
+enum { GX_PREC_FP64 = 0, GX_PREC_MIXED = 1 };
+
+typedef struct gx_ctx gx_ctx;
+
+/* Arguments of BaseManager::InitializeConstants (BaseManager.h:23) plus the
+ * knobs the reference hard-codes. */
+typedef struct gx_consts {
+    double half_width;      /* unused by Doppler/Voigt, kept for signature parity */
+    double temperature;     /* K */
+    double partition;       /* Q(T) */
+    double dfreq;           /* bin width of the grid passed to gx_transfer_freq */
+    double mean_mass;       /* amu */
+    double pressure;        /* atm (gamma arrives already multiplied by P) */
+    double ref_temp;        /* K, T_ref of the Lorentz temperature exponent */
+    double cutoff;          /* window half-width in cm^-1; <= 0: reference rule
+                               (Voigt min(25P,100), Doppler 100) */
+    double doppler_widths;  /* > 0: Doppler window k * alpha_i + dfreq per line */
+    int convention;         /* GX_CONV_* */
+    int precision;          /* GX_PREC_* */
+} gx_consts;
+
+/* Kernel times of the last gx_execute, from HIP events on the context's stream. */
+typedef struct gx_timing {
+    float intensity_ms;     /* line-intensity kernel */
+    float accumulate_ms;    /* bin-parallel profile accumulation kernel */
+    long long evals;        /* line-bin evaluations sum_i (ie_i - ib_i) of that call */
+    long long lines;        /* lines processed */
+} gx_timing;
+
+/* BaseManager(ProfileType) + GpuManager ctor (GPUManager.cpp:5-17): bind
+ * device `device`, take a memory budget (0: device total less 1 GiB). */
+int gx_create(int device, int profile, size_t mem_budget, gx_ctx **out);
+
+/* InitializeVectors(int Npoints) + GetNtrans() (GPUManager.cpp:49-76,
+ * BaseManager.cpp:25): allocate the grid and report the chunk capacity. */
+int gx_initialize_vectors(gx_ctx *ctx, int npoints, size_t *ntrans_out);
+
+/* InitializeConstants(...) (GPUManager.cpp:24-47, OpenMPManager.cpp:57-67). */
+int gx_initialize_constants(gx_ctx *ctx, const gx_consts *consts);
+
+/* TransferFreq(h_freq, h_intens, N) (GPUManager.cpp:78-86): grid nodes and
+ * the initial intensities.  Blocking. */
+int gx_transfer_freq(gx_ctx *ctx, const double *freq, const double *intens, int n);
+
+/* TransferVectors(Nener, E'', nu, A, g, gamma, n) (GPUManager.cpp:87-105):
+ * waits for prior work, copies the chunk (gamma/nexp NULL for Doppler), and
+ * sorts it by nu on the device when it is not already sorted.  Blocking. */
+int gx_transfer_vectors(gx_ctx *ctx, size_t nener, const double *energies, const double *nu, const double *aif,
+                        const int *gns, const double *gamma, const double *nexp);
+
+/* ExecuteCrossSection(N, N_ener, start_idx) (GPUManager.cpp:107-121): add
+ * the current chunk's first n_ener lines into bins [start_idx, start_idx+points)
+ * of the persistent grid.  Asynchronous. */
+int gx_execute(gx_ctx *ctx, int points, int n_ener, int start_idx);
+
+/* ReadyForWork() (GPUManager.cpp:185-188): 1 idle, 0 busy, <0 error. */
+int gx_ready(gx_ctx *ctx);
+
+/* TransferResults(h_freq, h_intens, N) (GPUManager.cpp:160-167): waits and
+ * overwrites h_intens (and h_freq when not NULL) with the device grid. */
+int gx_transfer_results(gx_ctx *ctx, double *freq, double *intens, int n);
+
+/* Cleanup() (GPUManager.cpp:172-183): free device buffers; the context stays valid. */
+int gx_cleanup(gx_ctx *ctx);
+
+/* Destroy the context (Cleanup + release of the stream and events). */
+int gx_destroy(gx_ctx *ctx);
+
+/* GetNtrans() (BaseManager.cpp:25). */
+size_t gx_get_ntrans(gx_ctx *ctx);
+
+/* Block until the context's stream is idle. */
+int gx_synchronize(gx_ctx *ctx);
+
+/* Kernel timing of the last gx_execute (synchronises). */
+int gx_last_timing(gx_ctx *ctx, gx_timing *out);
+
+/* Message of the last failing call on this host thread. */
+const char *gx_last_error(void);
+
+/* Library version string. */
+const char *gx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GX_H */

@@ -1,0 +1,178 @@
+"""Parity of the HIP library (through its C ABI) with the oracle and with the
+reference's own golden vectors.  Needs an MI355X.
+
+Bar: FP64 path within 1e-10 relative per bin for bins >= 1e-6 of the grid
+maximum; the mixed-precision path within 1e-5 of the FP64 oracle.
+"""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gpu_cross_amd import capi
+from gpu_cross_amd.manager import Mi355xManager, ProfileDriver, make_grid
+from gpu_cross_amd.synth import make_lines
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+FP64_TOL = 1e-10
+MIXED_TOL = 1e-5
+
+
+def rel_err(out, ref, floor=1e-6):
+    mask = ref >= floor * ref.max()
+    assert mask.sum() > 0
+    return float(np.max(np.abs(out[mask] - ref[mask]) / ref[mask]))
+
+
+def run_manager(profile, L, freq, d, T, Q, mass, P, points=None, start_idx=0, gamma=None, ref_temp=500.0, **kw):
+    points = freq.size if points is None else points
+    with Mi355xManager(profile, **kw) as m:
+        m.InitializeVectors(freq.size)
+        m.TransferFreq(freq, np.zeros_like(freq), freq.size)
+        m.InitializeConstants(0.0, T, Q, d, mass, P, ref_temp)
+        m.TransferVectors(L.n, L.elow, L.nu, L.aif, L.gns, gamma, L.nexp if gamma is not None else None)
+        m.ExecuteCrossSection(points, L.n, start_idx)
+        out = np.zeros_like(freq)
+        m.TransferResults(None, out, freq.size)
+        t = m.last_timing()
+    return out, t
+
+
+@pytest.fixture(scope="module")
+def voigt_lines():
+    ll = make_lines(2, 20000, 20000.0, 400_000, 12000.0, 17.03)
+    return ll
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_golden_through_driver(path):
+    z = np.load(path)
+    meta = json.loads(bytes(z["meta"]).decode())
+    print(f"golden {meta['name']}", flush=True)
+    voigt = meta["profile"] == capi.VOIGT
+    drv = ProfileDriver(meta["profile"], meta["start"], meta["end"], meta["npoints"], meta["T"], meta["Q"],
+                        meta["mass"], meta["P"], meta["ref_temp"], max_points=meta["max_points"],
+                        capacity=meta["capacity"])
+    out = drv.run(z["elow"], z["nu"], z["aif"], z["gns"], z["gamma"] if voigt else None,
+                  z["nexp"] if voigt else None)
+    assert rel_err(out, z["intens"]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("T,P,conv", [(1000.0, 1.0, 0), (296.0, 0.1, 0), (2500.0, 3.0, 0), (700.0, 0.5, 1)])
+def test_voigt_fp64_vs_oracle(voigt_lines, T, P, conv):
+    ll = voigt_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 2900.0) & (ll.nu < 3200.0))[0]
+    L = ll.subset(sel)                                # unsorted order: the library sorts on device
+    freq, d = make_grid(3000.0, 3100.0, 40000)
+    gam = L.gamma_at(P)
+    print(f"voigt fp64 T={T} P={P} conv={conv} lines={L.n}", flush=True)
+    out, t = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, convention=conv)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, convention=conv)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    assert rel_err(out, ref) <= FP64_TOL
+    assert t.evals == po.count_evals(c, L.nu, freq, freq.size, 0)
+
+
+def test_voigt_subrange_and_accumulation(voigt_lines):
+    """ExecuteCrossSection adds into bins [start_idx, start_idx+points) of a persistent grid."""
+    ll = voigt_lines
+    T, P = 1200.0, 1.0
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 4970.0) & (ll.nu < 5130.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(5000.0, 5100.0, 20000)
+    gam = L.gamma_at(P)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0)
+    with Mi355xManager(capi.VOIGT) as m:
+        m.InitializeVectors(freq.size)
+        m.TransferFreq(freq, np.zeros_like(freq), freq.size)
+        m.InitializeConstants(0.0, T, Q, d, 17.03, P, 500.0)
+        half = L.n // 2
+        first = L.subset(np.arange(half))
+        second = L.subset(np.arange(half, L.n))
+        m.TransferVectors(first.n, first.elow, first.nu, first.aif, first.gns, first.gamma_at(P), first.nexp)
+        m.ExecuteCrossSection(7000, first.n, 3000)
+        m.TransferVectors(second.n, second.elow, second.nu, second.aif, second.gns, second.gamma_at(P), second.nexp)
+        m.ExecuteCrossSection(12000, second.n, 8000)
+        out = np.zeros_like(freq)
+        m.TransferResults(None, out, freq.size)
+    ref = po.cross_section(c, first.elow, first.nu, first.aif, first.gns, first.gamma_at(P), first.nexp, freq, 7000,
+                           3000)
+    ref = po.cross_section(c, second.elow, second.nu, second.aif, second.gns, second.gamma_at(P), second.nexp, freq,
+                           12000, 8000, out=ref)
+    assert np.all(out[:3000] == 0.0)
+    assert rel_err(out, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("T,widths", [(296.0, 0.0), (1500.0, 0.0), (296.0, 25.0), (2000.0, 25.0)])
+def test_doppler_fp64_vs_oracle(T, widths):
+    ll = make_lines(1, 20000, 15000.0, 200_000, 10000.0, 30.01, voigt=False)
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 1900.0) & (ll.nu < 2200.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(2000.0, 2100.0, 20000)
+    print(f"doppler T={T} widths={widths} lines={L.n}", flush=True)
+    out, t = run_manager(capi.DOPPLER, L, freq, d, T, Q, 30.01, 1.0, ref_temp=296.0, doppler_widths=widths)
+    c = po.consts(po.DOPPLER, T, Q, d, 30.01, 1.0, 296.0)       # reference window: 100 cm^-1
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, None, None, freq, freq.size, 0, 4)
+    assert rel_err(out, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("T,P", [(1500.0, 0.1), (300.0, 1e-3), (3000.0, 10.0), (1000.0, 1.0)])
+def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
+    ll = voigt_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 5950.0) & (ll.nu < 6250.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(6000.0, 6200.0, 40000)
+    gam = L.gamma_at(P)
+    print(f"voigt mixed T={T} P={P} lines={L.n}", flush=True)
+    out, _ = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                         cutoff=25.0)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=25.0)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(out, ref)
+    print(f"  mixed max rel err {err:.3e}", flush=True)
+    assert err <= MIXED_TOL
+
+
+def test_fp64_deterministic_and_linear(voigt_lines):
+    ll = voigt_lines
+    T, P = 900.0, 1.0
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    freq, d = make_grid(0.0, 12000.0, 120000)
+    gam = ll.gamma_at(P)
+    a, _ = run_manager(capi.VOIGT, ll, freq, d, T, Q, 17.03, P, gamma=gam)
+    b, _ = run_manager(capi.VOIGT, ll, freq, d, T, Q, 17.03, P, gamma=gam)
+    assert np.array_equal(a, b)                                 # fixed summation order
+    ll2 = ll.subset(slice(None))
+    ll2.aif = ll.aif * 2.0
+    c2, _ = run_manager(capi.VOIGT, ll2, freq, d, T, Q, 17.03, P, gamma=gam)
+    assert np.array_equal(c2, 2.0 * a)                          # linear in S, bit for bit
+
+
+def test_edge_cases():
+    freq, d = make_grid(100.0, 110.0, 1000)
+    L = make_lines(4, 1000, 20000.0, 50, 12000.0, 17.03)
+    with Mi355xManager(capi.VOIGT) as m:
+        m.InitializeVectors(freq.size)
+        m.TransferFreq(freq, np.full(freq.size, 1.5), freq.size)      # non-zero initial grid is kept
+        m.InitializeConstants(0.0, 500.0, 100.0, d, 17.03, 1.0, 500.0)
+        m.TransferVectors(0, L.elow[:0], L.nu[:0], L.aif[:0], L.gns[:0], L.gamma_ref[:0], L.nexp[:0])
+        m.ExecuteCrossSection(freq.size, 0, 0)                         # empty chunk
+        m.TransferVectors(L.n, L.elow, L.nu, L.aif, L.gns, L.gamma_ref, L.nexp)
+        m.ExecuteCrossSection(0, L.n, 500)                             # empty bin range
+        out = np.zeros_like(freq)
+        m.TransferResults(None, out, freq.size)
+        assert np.all(out == 1.5)
+        with pytest.raises(capi.GxError):
+            m.ExecuteCrossSection(10, L.n, 995)                        # past the grid end
+        with pytest.raises(capi.GxError):
+            m.ExecuteCrossSection(10, L.n + 1, 0)                      # more lines than uploaded
+        assert m.ReadyForWork()
