@@ -42,8 +42,22 @@ class Consts(ctypes.Structure):
 
 
 class Timing(ctypes.Structure):
-    _fields_ = [("intensity_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float), ("evals", ctypes.c_longlong),
-                ("lines", ctypes.c_longlong)]
+    _fields_ = [("intensity_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
+                ("evals", ctypes.c_longlong), ("lines", ctypes.c_longlong)]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("start_nu", ctypes.c_double), ("end_nu", ctypes.c_double), ("npoints", ctypes.c_int)]
+
+
+class Point(ctypes.Structure):
+    _fields_ = [("temperature", ctypes.c_double), ("pressure", ctypes.c_double), ("partition", ctypes.c_double)]
+
+
+class TableOpts(ctypes.Structure):
+    _fields_ = [("profile", ctypes.c_int), ("precision", ctypes.c_int), ("convention", ctypes.c_int),
+                ("mean_mass", ctypes.c_double), ("ref_temp", ctypes.c_double), ("cutoff", ctypes.c_double),
+                ("doppler_widths", ctypes.c_double), ("use_graph", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 def build() -> str:
@@ -88,6 +102,17 @@ def load():
     _sig(L, "gx_get_ntrans", ctypes.c_size_t, [P])
     _sig(L, "gx_synchronize", ctypes.c_int, [P])
     _sig(L, "gx_last_timing", ctypes.c_int, [P, ctypes.POINTER(Timing)])
+    _sig(L, "gx_lines_create", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _ip, _dp, _dp,
+                                              ctypes.POINTER(P)])
+    _sig(L, "gx_lines_destroy", ctypes.c_int, [P])
+    _sig(L, "gx_lines_count", ctypes.c_size_t, [P])
+    _sig(L, "gx_table_create", ctypes.c_int, [P, ctypes.POINTER(Grid), ctypes.POINTER(Point), ctypes.c_int,
+                                              ctypes.POINTER(TableOpts), ctypes.POINTER(P)])
+    _sig(L, "gx_table_run", ctypes.c_int, [P])
+    _sig(L, "gx_table_wait", ctypes.c_int, [P, ctypes.POINTER(Timing)])
+    _sig(L, "gx_table_read", ctypes.c_int, [P, ctypes.c_int, _dp, ctypes.c_int])
+    _sig(L, "gx_table_shape", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)])
+    _sig(L, "gx_table_destroy", ctypes.c_int, [P])
     _sig(L, "gx_last_error", ctypes.c_char_p, [])
     _sig(L, "gx_version", ctypes.c_char_p, [])
     _lib = L

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -170,7 +171,9 @@ int LineStore::upload(size_t nlines, const double* h_e, const double* h_nu, cons
 
 int Derived::reserve(size_t n) {
   if (n <= cap) return GX_OK;
-  release();
+  hipFree(d.S); hipFree(d.g); hipFree(d.y); hipFree(d.ib); hipFree(d.ie);
+  d = DerivedF64{};
+  cap = 0;
   const size_t c = std::max<size_t>(n, 1);
   hipError_t e;
   if ((e = hipMalloc(&d.S, c * 8)) || (e = hipMalloc(&d.g, c * 8)) || (e = hipMalloc(&d.y, c * 8)) ||
@@ -180,10 +183,51 @@ int Derived::reserve(size_t n) {
   return GX_OK;
 }
 
+int Derived::reserve_mixed(size_t n, int npoints) {
+  if (n <= mcap && npoints <= mpoints) return GX_OK;
+  hipFree(m.amp); hipFree(m.y);
+  m = MixedLineData{};
+  mcap = 0;
+  mpoints = 0;
+  const size_t c = std::max<size_t>(n, 1), total = c * (size_t)std::max(npoints, 1);
+  hipError_t e;
+  if ((e = hipMalloc(&m.amp, total * 4)) || (e = hipMalloc(&m.y, total * 8)))
+    return hip_fail(e, "hipMalloc(mixed line data)");
+  m.stride = c;
+  mcap = c;
+  mpoints = std::max(npoints, 1);
+  return GX_OK;
+}
+
+int Derived::set_points(const PointConsts* pcs, int npts, hipStream_t s) {
+  hipError_t e;
+  const bool same = h_pcs.size() == (size_t)npts && std::memcmp(h_pcs.data(), pcs, sizeof(PointConsts) * npts) == 0;
+  if (same) return GX_OK;
+  if ((e = hipStreamSynchronize(s))) return hip_fail(e, "points: sync");
+  if (npts > pcs_cap) {
+    hipFree(d_pcs);
+    d_pcs = nullptr;
+    if ((e = hipMalloc(&d_pcs, sizeof(PointConsts) * (size_t)npts))) return hip_fail(e, "hipMalloc(points)");
+    pcs_cap = npts;
+  }
+  h_pcs.assign(pcs, pcs + npts);
+  if ((e = hipMemcpy(d_pcs, h_pcs.data(), sizeof(PointConsts) * (size_t)npts, hipMemcpyHostToDevice)))
+    return hip_fail(e, "points: upload");
+  return GX_OK;
+}
+
 void Derived::release() {
   hipFree(d.S); hipFree(d.g); hipFree(d.y); hipFree(d.ib); hipFree(d.ie);
   d = DerivedF64{};
   cap = 0;
+  hipFree(m.amp); hipFree(m.y);
+  m = MixedLineData{};
+  mcap = 0;
+  mpoints = 0;
+  hipFree(d_pcs);
+  d_pcs = nullptr;
+  pcs_cap = 0;
+  h_pcs.clear();
 }
 
 int Ranges::reserve(int nb) {
@@ -203,29 +247,129 @@ void Ranges::release() {
   cap = 0;
 }
 
+int MixedRunner::init() {
+  if (counter) return GX_OK;
+  // GX_MIXED_R (4 or 8) overrides the default bins per lane; used to compare launch shapes
+  if (const char* env = std::getenv("GX_MIXED_R")) {
+    const int r = std::atoi(env);
+    if (r == 4 || r == 8) R = r;
+  }
+  hipError_t e = hipMalloc(&counter, sizeof(unsigned));
+  if (e) return hip_fail(e, "hipMalloc(counter)");
+  int dev = 0;
+  if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)))
+    return hip_fail(e, "device attributes");
+  cus = std::max(cus, 1);
+  return GX_OK;
+}
+
+void MixedRunner::release() {
+  hipFree(d_tasks); hipFree(partial); hipFree(counter);
+  d_tasks = nullptr; partial = nullptr; counter = nullptr;
+  task_cap = 0; partial_cap = 0; cus = 0;
+  h_tasks.clear();
+}
+
+int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStream_t s) {
+  int rc = init();
+  if (rc) return rc;
+  plan = MixedPlan{};
+  if (nblocks <= 0 || ntasks <= 0) return GX_OK;
+  hipError_t e;
+  if (ntasks > task_cap) {
+    if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
+    h_tasks.clear();
+    hipFree(d_tasks);
+    d_tasks = nullptr;
+    if ((e = hipMalloc(&d_tasks, sizeof(MixedTask) * (size_t)ntasks))) return hip_fail(e, "hipMalloc(tasks)");
+    task_cap = ntasks;
+  }
+  const bool same = h_tasks.size() == (size_t)ntasks &&
+                    std::memcmp(h_tasks.data(), tasks, sizeof(MixedTask) * (size_t)ntasks) == 0;
+  if (!same) {
+    // the previous launch may still read the table: wait before overwriting it
+    if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
+    h_tasks.assign(tasks, tasks + ntasks);
+    if ((e = hipMemcpy(d_tasks, h_tasks.data(), sizeof(MixedTask) * (size_t)ntasks, hipMemcpyHostToDevice)))
+      return hip_fail(e, "mixed: task upload");
+  }
+  const int slots = cus * mixed_occupancy(R);
+  const long long base = (long long)ntasks * nblocks;
+  int nseg = forced_nseg;
+  if (nseg <= 0) {
+    const long long target = (long long)items_per_slot * slots;
+    nseg = (int)std::min<long long>(16, std::max<long long>(1, (target + base - 1) / base));
+  }
+  const long long nitems = base * nseg;
+  if (nitems > INT32_MAX) return fail(GX_INVALID_ARG, "mixed: too many work items");
+  if (nseg > 1) {
+    const size_t need = (size_t)nitems * (size_t)bins_per_block();
+    if (need > partial_cap) {
+      if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
+      hipFree(partial);
+      partial = nullptr;
+      if ((e = hipMalloc(&partial, need * sizeof(float2)))) return hip_fail(e, "hipMalloc(partials)");
+      partial_cap = need;
+    }
+  }
+  plan.nblocks = nblocks;
+  plan.nseg = nseg;
+  plan.ntasks = ntasks;
+  plan.tasks = d_tasks;
+  plan.partial = partial;
+  plan.counter = counter;
+  plan.nitems = (int)nitems;
+  plan.grid = (int)std::min<long long>(nitems, slots);
+  return GX_OK;
+}
+
+int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie,
+                        hipStream_t s) {
+  if (plan.nitems <= 0) return GX_OK;
+  plan.gv = gv;
+  plan.nu = nu;
+  plan.ranges = ranges;
+  plan.ib = ib;
+  plan.ie = ie;
+  hipError_t e = launch_accum_mixed(plan, R, s);
+  return e ? hip_fail(e, "mixed accumulate launch") : GX_OK;
+}
+
 // One (T,P) point over [start_idx, start_idx+points) of a grid: prep, ranges, accumulate.
 int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, int nlines, int precision,
-              Derived& der, Ranges& rng, double* intens, unsigned long long* d_evals, hipStream_t s,
-              hipEvent_t* ev) {
+              Derived& der, Ranges& rng, MixedRunner& mix, double* intens, unsigned long long* d_evals,
+              hipStream_t s, hipEvent_t* ev) {
   LinesView lv{ls.nu, ls.elow, ls.aif, ls.gns, ls.gam, ls.nexp, nlines};
   int rc = der.reserve((size_t)std::max(nlines, 1));
   if (rc) return rc;
   const bool mixed = precision == GX_PREC_MIXED && pc.profile == GX_VOIGT;
-  const int bpb = mixed ? mixed_bins_per_block() : kBinsF64;
+  const int bpb = mixed ? mix.bins_per_block() : kBinsF64;
   const int nblocks = gv.points > 0 ? (gv.points + bpb - 1) / bpb : 0;
   if ((rc = rng.reserve(nblocks))) return rc;
   hipError_t e;
+  if (mixed) {
+    if ((rc = der.reserve_mixed((size_t)std::max(nlines, 1), 1))) return rc;
+    if ((rc = der.set_points(&pc, 1, s))) return rc;
+    MixedTask task{der.m.amp, der.m.y, pc.dpwcoef, intens};
+    if ((rc = mix.prepare(&task, 1, nblocks, s))) return rc;
+  }
   if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");
-  if ((e = launch_line_prep_f64(pc, gv, lv, der.d, d_evals, s))) return hip_fail(e, "line prep launch");
+  if (mixed) {
+    const WindowArgs wa{pc.cutoff, pc.convention, 1};
+    if ((e = launch_windows(wa, gv, ls.nu, nlines, der.d.ib, der.d.ie, d_evals, s))) return hip_fail(e, "windows");
+    if ((e = launch_prep_mixed(der.d_pcs, 1, lv, der.m, s))) return hip_fail(e, "mixed prep launch");
+  } else {
+    if ((e = launch_line_prep_f64(pc, gv, lv, der.d, d_evals, s))) return hip_fail(e, "line prep launch");
+  }
   if (ev && (e = hipEventRecord(ev[1], s))) return hip_fail(e, "event");
   const double cut = max_cutoff(pc, ls.max_nu, gv.dfreq);
   if ((e = launch_block_ranges(gv, ls.nu, nlines, cut, bpb, nblocks, rng.r, s))) return hip_fail(e, "ranges launch");
   if (ev && (e = hipEventRecord(ev[2], s))) return hip_fail(e, "event");
-  if (mixed)
-    e = launch_accum_mixed(pc, gv, ls.nu, der.d, rng.r, nblocks, intens, s);
-  else
-    e = launch_accum_f64(pc, gv, ls.nu, der.d, rng.r, nblocks, intens, s);
-  if (e) return hip_fail(e, "accumulate launch");
+  if (mixed) {
+    if ((rc = mix.launch(gv, ls.nu, rng.r, der.d.ib, der.d.ie, s))) return rc;
+  } else {
+    if ((e = launch_accum_f64(pc, gv, ls.nu, der.d, rng.r, nblocks, intens, s))) return hip_fail(e, "accumulate launch");
+  }
   if (ev && (e = hipEventRecord(ev[3], s))) return hip_fail(e, "event");
   return GX_OK;
 }
@@ -251,7 +395,8 @@ struct gx_ctx {
   LineStore lines;
   Derived derived;
   Ranges ranges;
-  unsigned long long* d_evals = nullptr;
+  MixedRunner mix;
+  unsigned long long* d_evals = nullptr;   // kEvalSlots counters
   int last_lines = 0;
 };
 
@@ -289,7 +434,7 @@ int gx_create(int device, int profile, size_t mem_budget, gx_ctx** out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) { delete c; return hip_fail(e, "stream"); }
   for (auto& ev : c->ev)
     if ((e = hipEventCreate(&ev))) { gx_destroy(c); return hip_fail(e, "event"); }
-  if ((e = hipMalloc(&c->d_evals, sizeof(unsigned long long)))) { gx_destroy(c); return hip_fail(e, "hipMalloc"); }
+  if ((e = hipMalloc(&c->d_evals, sizeof(unsigned long long) * kEvalSlots))) { gx_destroy(c); return hip_fail(e, "hipMalloc"); }
   *out = c;
   return GX_OK;
 }
@@ -375,11 +520,11 @@ int gx_execute(gx_ctx* c, int points, int n_ener, int start_idx) {
   const gx_consts& k = c->consts;
   PointConsts pc = make_point_consts(c->profile, k, 1.0);
   GridView gv{c->d_freq, c->M, start_idx, points, c->start_nu, k.dfreq};
-  hipError_t e = hipMemsetAsync(c->d_evals, 0, sizeof(unsigned long long), c->stream);
+  hipError_t e = hipMemsetAsync(c->d_evals, 0, sizeof(unsigned long long) * kEvalSlots, c->stream);
   if (e) return hip_fail(e, "execute: memset");
   c->last_lines = n_ener;
   c->timed = true;
-  return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->d_intens, c->d_evals, c->stream,
+  return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->mix, c->d_intens, c->d_evals, c->stream,
                    c->ev);
 }
 
@@ -416,6 +561,7 @@ int gx_cleanup(gx_ctx* c) {
   c->lines.release();
   c->derived.release();
   c->ranges.release();
+  c->mix.release();
   return GX_OK;
 }
 
@@ -445,10 +591,14 @@ int gx_last_timing(gx_ctx* c, gx_timing* out) {
   float a = 0, b = 0;
   if ((e = hipEventElapsedTime(&a, c->ev[0], c->ev[1])) || (e = hipEventElapsedTime(&b, c->ev[2], c->ev[3])))
     return hip_fail(e, "last_timing: events");
-  unsigned long long ev = 0;
-  if ((e = hipMemcpy(&ev, c->d_evals, sizeof(ev), hipMemcpyDeviceToHost))) return hip_fail(e, "last_timing: copy");
+  unsigned long long slots[kEvalSlots], ev = 0;
+  if ((e = hipMemcpy(slots, c->d_evals, sizeof(slots), hipMemcpyDeviceToHost))) return hip_fail(e, "last_timing: copy");
+  for (unsigned long long v : slots) ev += v;
+  float tot = 0;
+  if ((e = hipEventElapsedTime(&tot, c->ev[0], c->ev[3]))) return hip_fail(e, "last_timing: events");
   out->intensity_ms = a;
   out->accumulate_ms = b;
+  out->total_ms = tot;
   out->evals = (long long)ev;
   out->lines = c->last_lines;
   return GX_OK;

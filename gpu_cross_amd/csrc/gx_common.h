@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/gx.h"
 #include "gx_internal.h"
@@ -38,11 +39,22 @@ struct LineStore {
              const double* nexp, hipStream_t s);
 };
 
-// Per-point derived line data (FP64).
+// Per-point derived line data: FP64 arrays of the plain path (whose ib/ie
+// also serve as the shared windows of the mixed path) and FP32 arrays of the
+// mixed path for up to `points_cap` points.
 struct Derived {
   DerivedF64 d{};
   size_t cap = 0;
+  MixedLineData m{};
+  size_t mcap = 0;         // lines per point
+  int mpoints = 0;         // points
+  PointConsts* d_pcs = nullptr;
+  int pcs_cap = 0;
+  std::vector<PointConsts> h_pcs;
   int reserve(size_t n);
+  int reserve_mixed(size_t n, int npoints);
+  // upload per-point constants (waits for the stream only when they change)
+  int set_points(const PointConsts* pcs, int npts, hipStream_t s);
   void release();
 };
 
@@ -54,9 +66,35 @@ struct Ranges {
   void release();
 };
 
+// Launch state of the mixed-precision kernel: task table, segment partials,
+// work-queue counter.  Host copies of the task table stay alive here until the
+// next run so asynchronous uploads never read a dead buffer.
+struct MixedRunner {
+  MixedTask* d_tasks = nullptr;
+  int task_cap = 0;
+  float2* partial = nullptr;
+  size_t partial_cap = 0;      // float2 elements
+  unsigned* counter = nullptr;
+  int cus = 0;
+  int R = 4;                   // bins per lane (4 or 8)
+  int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
+  int forced_nseg = 0;         // > 0: fixed segments per block
+  std::vector<MixedTask> h_tasks;
+  MixedPlan plan{};
+  int init();
+  int bins_per_block() const { return mixed_bins_per_block(R); }
+  // Upload the task table, size the partials and fix the launch shape.
+  // May synchronise the stream; never call it inside a graph capture.
+  int prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStream_t s);
+  // Enqueue the kernels of the prepared plan (graph-capturable).
+  int launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie, hipStream_t s);
+  void release();
+};
+
 // Launch line prep, block ranges and accumulation of one point (asynchronous).
 // ev, when not NULL, holds 4 events recorded around the kernels.
 int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, int nlines, int precision,
-              Derived& der, Ranges& rng, double* intens, unsigned long long* d_evals, hipStream_t s, hipEvent_t* ev);
+              Derived& der, Ranges& rng, MixedRunner& mix, double* intens, unsigned long long* d_evals,
+              hipStream_t s, hipEvent_t* ev);
 
 }  // namespace gx

@@ -55,6 +55,49 @@ struct DerivedF64 {
   int* ie;
 };
 
+// Per-point line data of the mixed-precision path, [npoints][stride].  1/alpha
+// is recomputed in FP64 at staging (one division), and y is kept in FP64, so
+// every W4 region decision is the FP64 path's own.
+struct MixedLineData {
+  float* amp;     // S / alpha (FP32)
+  double* y;      // gamma_T / alpha
+  size_t stride;  // lines per point
+};
+
+// Window rule shared by all points of a launch.
+struct WindowArgs {
+  double cutoff;
+  int convention;     // 0 round() rule, 1 per-bin |nu - f| <= C
+  int zero_to_tiny;   // nu == 0 -> 1e-6 (OpenMPManager.cpp:198)
+};
+
+// One (T,P) point of a mixed-precision launch: its line data and its grid.
+struct MixedTask {
+  const float* amp;
+  const double* y;
+  double dpwcoef;     // Doppler width per unit nu of this point
+  double* out;
+};
+
+// A mixed-precision launch over ntasks points sharing one grid, one sorted
+// line list and one set of block ranges.  Work items are (task, block, segment).
+struct MixedPlan {
+  GridView gv;
+  const double* nu;        // sorted line centres
+  const int2* ranges;      // per-block line slices
+  const int* ib;           // shared line windows
+  const int* ie;
+  int nblocks;             // bin blocks of mixed_bins_per_block() bins
+  int nseg;                // line segments per block (partials reduced in order when > 1)
+  int ntasks;
+  const MixedTask* tasks;  // device array [ntasks]
+  float2* partial;         // [nitems][bins per block] when nseg > 1
+  unsigned* counter;       // work-queue head, reset by the launch
+  int nitems;              // ntasks * nblocks * nseg
+  int grid;                // persistent workgroups
+};
+
+constexpr int kEvalSlots = 64;       // eval counters (spread to avoid one hot atomic)
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kTile = 256;           // lines staged in LDS per pass
 
@@ -71,11 +114,17 @@ hipError_t launch_block_ranges(const GridView& gv, const double* nu, int n, doub
                                int nblocks, int2* ranges, hipStream_t s);
 hipError_t launch_accum_f64(const PointConsts& pc, const GridView& gv, const double* nu_sorted, DerivedF64 d,
                             const int2* ranges, int nblocks, double* intens, hipStream_t s);
-hipError_t launch_accum_mixed(const PointConsts& pc, const GridView& gv, const double* nu_sorted, DerivedF64 d,
-                              const int2* ranges, int nblocks, double* intens, hipStream_t s);
+hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double* nu, int n, int* ib, int* ie,
+                          unsigned long long* evals, hipStream_t s);
+hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, MixedLineData out,
+                             hipStream_t s);
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
+// R = bins per lane (4 or 8); a workgroup covers mixed_bins_per_block(R) bins
+hipError_t launch_accum_mixed(const MixedPlan& plan, int R, hipStream_t s);
+int mixed_occupancy(int R);   // resident workgroups per CU of the mixed kernel
+int mixed_bins_per_block(int R);
 
-// bins covered by one workgroup of the accumulation kernels
+// bins covered by one workgroup of the FP64 accumulation kernels
 constexpr int kBinsF64 = 256;
-int mixed_bins_per_block();
 
 }  // namespace gx

@@ -8,10 +8,12 @@
 //               slice through LDS in 256-line tiles (coalesced SoA loads), and every lane
 //               sums its own bins in line order, so each grid is deterministic.  No atomics.
 //
-// Workgroups are mapped XCD-aware: hardware dispatches block b to XCD b % 8, and the remap
-// gives each XCD one contiguous range of bin blocks so neighbouring blocks, which stream
-// almost the same lines, share that XCD's L2.
+// Bin blocks are not remapped per XCD: the hardware deals blocks round-robin to the 8 XCDs,
+// which interleaves dense and sparse parts of the spectrum evenly, while a block's line
+// slice is streamed once per block at ~0.1 B per evaluation, far from any cache limit.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -23,15 +25,10 @@ namespace gx {
 namespace {
 
 constexpr double kIsqrtPi = 0.564189584;  // defines.h:8 (the truncated literal the reference uses)
-constexpr int kXcds = 8;
 
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  const int xcd = bid % kXcds, idx = bid / kXcds;
-  const int per = nblocks / kXcds, rem = nblocks % kXcds;
-  return xcd * per + min(xcd, rem) + idx;
+__global__ void k_fill(double* __restrict__ p, size_t n, double v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
-
-__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __global__ void k_iota(uint32_t* idx, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,6 +71,19 @@ __device__ int first_lt(double nu, double bound, const double* __restrict__ f, i
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
+}
+
+// Sum a per-thread count over a 256-thread block, then one atomic per block
+// into one of kEvalSlots counters (spread so no single address serialises).
+__device__ __forceinline__ void block_count(unsigned long long w, unsigned long long* evals) {
+  __shared__ unsigned long long s_w[4];
+  w = wave_sum_u64(w);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (tot) atomicAdd(&evals[blockIdx.x & (kEvalSlots - 1)], tot);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -129,10 +139,63 @@ __global__ __launch_bounds__(256) void k_line_prep(PointConsts pc, GridView gv, 
     out.ie[i] = ie;
     w = (unsigned long long)(ie - ib);
   }
-  if (evals) {
-    w = wave_sum_u64(w);
-    if ((threadIdx.x & 63) == 0 && w) atomicAdd(evals, w);
+  if (evals) block_count(w, evals);
+}
+
+// ---------------------------------------------------------------------------
+// Integer windows [ib, ie) of every line for one chunk and one cutoff: they
+// depend on the grid and the cutoff only, so all (T,P) points with the same
+// cutoff share them.  Same expressions as k_line_prep (OpenMPManager.cpp:205-206).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_windows(WindowArgs wa, GridView gv, const double* __restrict__ nu_s, int n,
+                                                 int* __restrict__ IB, int* __restrict__ IE,
+                                                 unsigned long long* __restrict__ evals) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long w = 0;
+  if (i < n) {
+    const double nu = nu_s[i];
+    const double nu_if = (nu == 0 && wa.zero_to_tiny) ? 1e-6 : nu;
+    const int s0 = gv.start_idx, s1 = gv.start_idx + gv.points;
+    int ib, ie;
+    if (wa.convention == 1) {
+      ib = first_le(nu_if, wa.cutoff, gv.freq, s0, s1);
+      ie = first_lt(nu_if, -wa.cutoff, gv.freq, s0, s1);
+    } else {
+      ib = (int)fmax(round((nu_if - wa.cutoff - gv.start_nu) / gv.dfreq), (double)s0);
+      ie = (int)fmin(round((nu_if + wa.cutoff - gv.start_nu) / gv.dfreq), (double)s1);
+    }
+    if (ie < ib) ie = ib;
+    IB[i] = ib;
+    IE[i] = ie;
+    w = (unsigned long long)(ie - ib);
   }
+  if (evals) block_count(w, evals);
+}
+
+// ---------------------------------------------------------------------------
+// Per-point line data of the mixed-precision path: S, 1/alpha and y in FP64
+// exactly as k_line_prep, stored as FP32 (amp = S/alpha).  blockIdx.y = point.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, LinesView lv,
+                                                    MixedLineData out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (i >= lv.n) return;
+  const PointConsts pc = pcs[p];
+  const double nu = lv.nu[i];
+  const double nu_if = nu == 0 ? 1e-6 : nu;
+  const double a = lv.aif[i];
+  const int gi = lv.gns[i];
+  double S = pc.cmcoef * a * gi * kIsqrtPi * exp(-pc.beta * lv.elow[i]) * (1.0 - exp(-pc.beta * nu_if)) /
+             (nu_if * nu_if * pc.partition);
+  if (pc.convention == 1 && gi == -1) S = a;
+  const double g = 1.0 / (nu_if * pc.dpwcoef);
+  const double gammaL = (lv.gamma[i] * pc.pressure_scale) * pow(pc.ref_temp / pc.temperature, lv.nexp[i]);
+  const size_t o = (size_t)p * out.stride + i;
+  out.amp[o] = (float)(S * g);
+  out.y[o] = gammaL * g;
 }
 
 // ---------------------------------------------------------------------------
@@ -175,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
   __shared__ double s_nu[kTile], s_S[kTile], s_g[kTile], s_y[kTile];
   __shared__ int s_ib[kTile], s_ie[kTile];
   const int t = threadIdx.x;
-  const int b = xcd_remap(blockIdx.x, nblocks);
+  const int b = blockIdx.x;
   const int j = gv.start_idx + b * kBinsF64 + t;
   const bool active = j < gv.start_idx + gv.points;
   const double f = active ? gv.freq[j] : 0.0;
@@ -217,167 +280,274 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
 }
 
 // ---------------------------------------------------------------------------
-// Mixed-precision Voigt accumulation.
+// Mixed-precision Voigt accumulation (persistent, work-queue driven).
 //
-// A workgroup owns kBinsMixed = 4 waves x 64 lanes x R bins; lane l of wave w
-// sums bins w0 + l + 64q (q < R), w0 = j0 + 256 w.  Per staged line (FP64 -> LDS):
-//   c      nearest bin to the centre, relative to the block origin j0
-//   r      (nu - f_{j0+c}) / alpha   (|r| <= e/2, FP32)
-//   e      dfreq / alpha             (FP32)
-// so x_j = r - (j - j0 - c) e with an exact integer factor: the FP32 offset
-// error stays relative to x itself (no large-number cancellation).
-// Region I of W4 is evaluated in FP32 in the closed form
+// Work item = (point, bin block, line segment).  A workgroup of 4 waves owns
+// kBinsMixed = 4 x 64 x kR contiguous bins; lane l of wave w sums bins
+// w0 + l + 64q (q < kR), w0 = j0 + 64 kR w.  Its line segment is streamed
+// through LDS in 256-line tiles; per staged line (FP64 -> LDS):
+//   c      nearest bin to the centre (kept as an exact float integer)
+//   r      (nu - f_{c}) / alpha      (|r| <= e/2)
+//   e      dfreq / alpha
+// so x_j = r - (j - c) e with an exact integer factor and the FP32 offset
+// error stays relative to x itself.  Region I of W4 runs in FP32 as
 //   (A1 + A2 x^2) / (x^2 (x^2 + R) + P2),
-//   A1 = amp c y (0.5+y^2), A2 = amp c y, R = 2y^2 - 1, P2 = (0.5+y^2)^2.
-// Bins with |x| + y < 15 (the core, found per line in FP64 as an integer
-// range so the region choice equals the FP64 path's) are evaluated by the FP64
-// W4.  Window and core tests are wave-uniform per line: a wave whose 256 bins
-// lie inside the window and outside the core runs the branch-free FP32 loop.
+//   A1 = amp c y (0.5+y^2), A2 = amp c y, R = 2y^2 - 1, P2 = (0.5+y^2)^2,
+// and bins with |x| + y < 15 (the core, found per line in FP64 as an integer
+// range so the region choice equals the FP64 path's) go to the FP64 W4.
+// Window and core tests are scalar per (wave, line): a wave whose bins all
+// lie inside the window and outside the core runs the branch-free loop.
 // Per-bin sums: plain FP32 over 32 lines, then a Kahan step into (sum, comp).
+// Segments of one block land in partial buffers and are added in segment
+// order by k_reduce_segments, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
-constexpr int kR = 4;
-constexpr int kBinsMixed = kBlock * kR;
+template <int R>
+struct MixedShape {
+  static constexpr int kWaveBins = 64 * R;
+  static constexpr int kBins = 4 * kWaveBins;
+};
 
-__global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(GridView gv, const double* __restrict__ nu_s,
-                                                              const double* __restrict__ S,
-                                                              const double* __restrict__ G,
-                                                              const double* __restrict__ Y,
-                                                              const int* __restrict__ IB, const int* __restrict__ IE,
-                                                              const int2* __restrict__ ranges, int nblocks,
-                                                              double* __restrict__ intens, double zero_nu) {
-  __shared__ float s_r[kTile], s_e[kTile], s_a1[kTile], s_a2[kTile], s_rr[kTile], s_p2[kTile], s_amp[kTile];
-  __shared__ int s_c[kTile], s_ib[kTile], s_ie[kTile], s_k0[kTile], s_k1[kTile];
+struct alignas(16) StagedLine {
+  int ib, ie, k0, k1;         // window and core ranges (absolute bins)
+  float r, e, a1, a2;         // fast-path operands
+  float rr, p2, pad, cf;      // cf: centre bin relative to the block origin
+};
+
+__device__ __forceinline__ void kahan_step(float& sum, float& comp, float& part) {
+  const float yk = part - comp;
+  const float tk = sum + yk;
+  comp = (tk - sum) - yk;
+  sum = tk;
+  part = 0.0f;
+}
+
+__device__ __forceinline__ float region1(float x, float a1, float a2, float rr, float p2) {
+  const float x2 = x * x;
+  const float num = __builtin_fmaf(a2, x2, a1);
+  const float den = __builtin_fmaf(x2, x2 + rr, p2);
+  return num * __builtin_amdgcn_rcpf(den);
+}
+
+__global__ void k_reset_counter(unsigned* c) { *c = 0u; }
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
+  constexpr int kWaveBins = MixedShape<R>::kWaveBins;
+  constexpr int kBinsMixed = MixedShape<R>::kBins;
+  __shared__ StagedLine s_l[kTile];
   __shared__ double s_nu[kTile], s_g[kTile], s_y[kTile];
+  __shared__ float s_amp[kTile];
+  __shared__ int s_item;
 
   const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int b = xcd_remap(blockIdx.x, nblocks);
-  const int j0 = gv.start_idx + b * kBinsMixed;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const GridView gv = P.gv;
   const int jend = gv.start_idx + gv.points;
-  const int w0 = j0 + wave * 256;
-  const int2 r = ranges[b];
+  const int per_task = P.nblocks * P.nseg;
 
-  float sum[kR], comp[kR], part[kR];
-#pragma unroll
-  for (int q = 0; q < kR; ++q) sum[q] = comp[q] = part[q] = 0.0f;
-
-  for (int base = r.x; base < r.y; base += kTile) {
-    const int i = base + t;
-    if (i < r.y) {
-#pragma clang fp contract(off)
-      // FP64 staging of one line
-      const double v0 = nu_s[i];
-      const double nu = v0 == 0.0 ? zero_nu : v0;
-      const double g = G[i], y = Y[i];
-      const double amp = S[i] * g;
-      const double pos = (nu - gv.start_nu) / gv.dfreq;  // bin coordinate of the centre
-      const double cj = rint(pos);
-      const double fc = gv.start_nu + cj * gv.dfreq;
-      s_c[t] = (int)cj - j0;
-      s_r[t] = (float)((nu - fc) * g);
-      s_e[t] = (float)(gv.dfreq * g);
-      const double y2 = y * y;
-      s_a2[t] = (float)(amp * 0.5641896 * y);
-      s_a1[t] = (float)(amp * 0.5641896 * y * (0.5 + y2));
-      s_rr[t] = (float)(2.0 * y2 - 1.0);
-      s_p2[t] = (float)((0.5 + y2) * (0.5 + y2));
-      s_amp[t] = (float)amp;
-      s_ib[t] = IB[i];
-      s_ie[t] = IE[i];
-      s_nu[t] = nu;
-      s_g[t] = g;
-      s_y[t] = y;
-      // core bins: |nu - f_j| g + y < 15, as an exact integer range [k0, k1)
-      int k0 = 0, k1 = 0;  // empty
-      if (y < 15.0) {
-        const double half = (15.0 - y) / g;
-        int a = (int)floor((nu - half - gv.start_nu) / gv.dfreq) - 1;
-        int bnd = (int)ceil((nu + half - gv.start_nu) / gv.dfreq) + 1;
-        auto core = [&](int jj) {
-          const double fj = gv.start_nu + (double)jj * gv.dfreq;
-          return fabs(nu - fj) * g + y < 15.0;
-        };
-        while (a < bnd && !core(a)) ++a;
-        while (bnd > a && !core(bnd - 1)) --bnd;
-        k0 = a;
-        k1 = bnd;
-      }
-      s_k0[t] = k0;
-      s_k1[t] = k1;
-    }
+  for (;;) {
+    if (t == 0) s_item = (int)atomicAdd(P.counter, 1u);
     __syncthreads();
-    const int cnt = min(kTile, r.y - base);
-    for (int k = 0; k < cnt; ++k) {
-      const int ib = uniform_int(s_ib[k]), ie = uniform_int(s_ie[k]);
-      if (ie > w0 && ib < w0 + 256) {
-        const int c = uniform_int(s_c[k]);
-        const int k0 = uniform_int(s_k0[k]), k1 = uniform_int(s_k1[k]);
-        const float e = s_e[k], rr0 = s_r[k];
-        const float e64 = 64.0f * e;
-        // x for bin q of this lane: r - (w0 - j0 + lane + 64 q - c) e
-        const float x0 = __builtin_fmaf(-(float)(w0 - j0 + lane - c), e, rr0);
-        const float a1 = s_a1[k], a2 = s_a2[k], rr = s_rr[k], p2 = s_p2[k];
-        const bool fast = ib <= w0 && ie >= w0 + 256 && (k0 >= k1 || k1 <= w0 || k0 >= w0 + 256);
-        if (fast) {
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= P.nitems) break;
+    const int task = item / per_task;
+    const int rem = item - task * per_task;
+    const int b = rem / P.nseg;
+    const int seg = rem - b * P.nseg;
+    const MixedTask tk = P.tasks[task];
+    const int j0 = gv.start_idx + b * kBinsMixed;
+    const int w0 = j0 + wave * kWaveBins;
+    const int w1 = w0 + kWaveBins;
+    // bin offsets of this lane from the block origin, exact in FP32: x = (c - j) e + r
+    // is formed from the exact integer c - j, so its error stays relative to x itself
+    float lanef_q[R];
 #pragma unroll
-          for (int q = 0; q < kR; ++q) {
-            const float x = __builtin_fmaf(-(float)q, e64, x0);
-            const float x2 = x * x;
-            const float num = __builtin_fmaf(a2, x2, a1);
-            const float den = __builtin_fmaf(x2, x2 + rr, p2);
-            part[q] += num * __builtin_amdgcn_rcpf(den);
+    for (int q = 0; q < R; ++q) lanef_q[q] = (float)(w0 - j0 + lane + 64 * q);
+    const int2 rg = P.ranges[b];
+    const int len = rg.y - rg.x;
+    const int lo = rg.x + (int)(((long long)len * seg) / P.nseg);
+    const int hi = rg.x + (int)(((long long)len * (seg + 1)) / P.nseg);
+
+    float sum[R], comp[R], part[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) sum[q] = comp[q] = part[q] = 0.0f;
+    int since_flush = 0;
+
+    for (int base = lo; base < hi; base += kTile) {
+      const int i = base + t;
+      if (i < hi) {
+#pragma clang fp contract(off)
+        const double v0 = P.nu[i];
+        const double nu = v0 == 0.0 ? 1e-6 : v0;
+        const double g = 1.0 / (nu * tk.dpwcoef), y = tk.y[i];     // as k_line_prep: same bits
+        const double cj = rint((nu - gv.start_nu) / gv.dfreq);      // centre bin
+        const double fc = gv.start_nu + cj * gv.dfreq;
+        StagedLine L;
+        L.ib = P.ib[i];
+        L.ie = P.ie[i];
+        const float amp = tk.amp[i];
+        // Power-of-two rescaling x' = sigma x keeps x'^4 far from FP32 overflow for lines
+        // whose window spans huge x (tiny nu => huge 1/alpha); region I is invariant under
+        // (A1, A2, R, P2) -> (A1 s^4, A2 s^2, R s^2, P2 s^4).  sigma = 1 for ordinary lines.
+        const double xmax = (fabs((double)L.ib - cj) + fabs((double)L.ie - cj) + 2.0) * gv.dfreq * g;
+        const int kexp = xmax > 65536.0 ? (int)ceil(log2(xmax / 65536.0)) : 0;
+        const double sg = ldexp(1.0, -kexp), sg2 = sg * sg, sg4 = sg2 * sg2;
+        const float e = (float)(gv.dfreq * g * sg);
+        L.r = (float)((nu - fc) * g * sg);
+        L.e = e;
+        L.pad = 0.0f;
+        const double y2 = y * y;
+        L.a2 = (float)((double)amp * 0.5641896 * y * sg2);
+        L.a1 = (float)((double)amp * 0.5641896 * y * (0.5 + y2) * sg4);
+        L.rr = (float)((2.0 * y2 - 1.0) * sg2);
+        L.p2 = (float)((0.5 + y2) * (0.5 + y2) * sg4);
+        L.cf = (float)(cj - (double)j0);
+        // core bins: |nu - f_j| g + y < 15, as an exact integer range [k0, k1)
+        int k0 = 0, k1 = 0;
+        if (y < 15.0 && L.ie > L.ib) {
+          const double half = (15.0 - y) / g;
+          int a = (int)floor((nu - half - gv.start_nu) / gv.dfreq) - 1;
+          int z = (int)ceil((nu + half - gv.start_nu) / gv.dfreq) + 1;
+          a = max(a, L.ib);
+          z = min(z, L.ie);
+          auto core = [&](int jj) {
+            const double fj = gv.start_nu + (double)jj * gv.dfreq;
+            return fabs(nu - fj) * g + y < 15.0;
+          };
+          while (a < z && !core(a)) ++a;
+          while (z > a && !core(z - 1)) --z;
+          k0 = a;
+          k1 = z;
+        }
+        L.k0 = k0;
+        L.k1 = k1;
+        s_l[t] = L;
+        s_nu[t] = nu;
+        s_g[t] = g;
+        s_y[t] = y;
+        s_amp[t] = amp;
+      }
+      __syncthreads();
+      const int cnt = min(kTile, hi - base);
+      // Wave-level pruning: one ballot per 64 staged lines splits them into
+      //   fast  -- window covers all of this wave's bins, no W4 core among them
+      //   edge  -- touches the wave otherwise (partial window or core bins)
+      // and lines that miss the wave entirely are dropped.
+      unsigned long long fast_m[kTile / 64], edge_m[kTile / 64];
+#pragma unroll
+      for (int m = 0; m < kTile / 64; ++m) {
+        const int k = m * 64 + lane;
+        bool f = false, ed = false;
+        if (k < cnt) {
+          const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
+          const bool touch = I.y > w0 && I.x < w1;
+          const bool full = I.x <= w0 && I.y >= w1;
+          const bool core = I.w > I.z && I.w > w0 && I.z < w1;
+          f = full && !core;
+          ed = touch && !f;
+        }
+        fast_m[m] = __ballot(f);
+        edge_m[m] = __ballot(ed);
+      }
+      // fast lines: branch-free FP32 region I for all R bins of every lane.  The next
+      // line's record is read while this one is evaluated.
+#pragma unroll
+      for (int m = 0; m < kTile / 64; ++m) {
+        unsigned long long bits = fast_m[m];
+        if (!bits) continue;
+        int k = m * 64 + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        float4 F = *reinterpret_cast<const float4*>(&s_l[k].r);
+        float4 G = *reinterpret_cast<const float4*>(&s_l[k].rr);
+        for (;;) {
+          const bool more = bits != 0;
+          const int kn = more ? m * 64 + __builtin_ctzll(bits) : k;
+          const float4 Fn = *reinterpret_cast<const float4*>(&s_l[kn].r);
+          const float4 Gn = *reinterpret_cast<const float4*>(&s_l[kn].rr);
+#pragma unroll
+          for (int q = 0; q < R; ++q)
+            part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
+          if (++since_flush == 32) {
+            since_flush = 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
           }
-        } else {
+          if (!more) break;
+          bits &= bits - 1;
+          F = Fn;
+          G = Gn;
+        }
+      }
+      // edge lines: per-lane window and core tests; core bins in FP64
 #pragma unroll
-          for (int q = 0; q < kR; ++q) {
+      for (int m = 0; m < kTile / 64; ++m) {
+        unsigned long long bits = edge_m[m];
+        while (bits) {
+          const int k = m * 64 + __builtin_ctzll(bits);
+          bits &= bits - 1;
+          const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
+          const float4 F = *reinterpret_cast<const float4*>(&s_l[k].r);
+          const float4 G = *reinterpret_cast<const float4*>(&s_l[k].rr);
+#pragma unroll
+          for (int q = 0; q < R; ++q) {
             const int jj = w0 + lane + 64 * q;
-            if (jj >= ib && jj < ie) {
-              if (jj >= k0 && jj < k1) {
-                const double f = gv.freq[jj];
-                const double x = fabs(s_nu[k] - f) * s_g[k];
+            if (jj >= I.x && jj < I.y) {
+              if (jj >= I.z && jj < I.w) {
+                const double x = fabs(s_nu[k] - gv.freq[jj]) * s_g[k];
                 part[q] += s_amp[k] * (float)w4_inner(x, s_y[k]);
               } else {
-                const float x = __builtin_fmaf(-(float)q, e64, x0);
-                const float x2 = x * x;
-                const float num = __builtin_fmaf(a2, x2, a1);
-                const float den = __builtin_fmaf(x2, x2 + rr, p2);
-                part[q] += num * __builtin_amdgcn_rcpf(den);
+                part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
               }
             }
           }
+          if (++since_flush == 32) {
+            since_flush = 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+          }
         }
       }
-      if ((k & 31) == 31) {  // compensated step every 32 lines
+      __syncthreads();
+    }
 #pragma unroll
-        for (int q = 0; q < kR; ++q) {
-          const float yk = part[q] - comp[q];
-          const float tk = sum[q] + yk;
-          comp[q] = (tk - sum[q]) - yk;
-          sum[q] = tk;
-          part[q] = 0.0f;
-        }
+    for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+    if (P.nseg == 1) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int jj = w0 + lane + 64 * q;
+        if (jj < jend) tk.out[jj] += (double)sum[q] - (double)comp[q];
       }
-    }
+    } else {
+      float2* dst = P.partial + (size_t)item * kBinsMixed;
 #pragma unroll
-    for (int q = 0; q < kR; ++q) {
-      const float yk = part[q] - comp[q];
-      const float tk = sum[q] + yk;
-      comp[q] = (tk - sum[q]) - yk;
-      sum[q] = tk;
-      part[q] = 0.0f;
+      for (int q = 0; q < R; ++q) dst[wave * kWaveBins + lane + 64 * q] = make_float2(sum[q], comp[q]);
     }
-    __syncthreads();
   }
-#pragma unroll
-  for (int q = 0; q < kR; ++q) {
-    const int jj = w0 + lane + 64 * q;
-    if (jj < jend) intens[jj] += (double)sum[q] - (double)comp[q];
+}
+
+// out[j] += sum over segments, in segment order, of (sum - comp)
+__global__ void k_reduce_segments(MixedPlan P, int kBinsMixed) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per_task = (long long)P.nblocks * kBinsMixed;
+  if (idx >= per_task * P.ntasks) return;
+  const int task = (int)(idx / per_task);
+  const int off = (int)(idx - task * per_task);
+  const int b = off / kBinsMixed, m = off - b * kBinsMixed;
+  const int j = P.gv.start_idx + off;
+  if (j >= P.gv.start_idx + P.gv.points) return;
+  const size_t item0 = ((size_t)task * P.nblocks + b) * P.nseg;
+  double acc = 0.0;
+  for (int sg = 0; sg < P.nseg; ++sg) {
+    const float2 v = P.partial[(item0 + sg) * kBinsMixed + m];
+    acc += (double)v.x - (double)v.y;
   }
+  P.tasks[task].out[j] += acc;
 }
 
 }  // namespace
 
-int mixed_bins_per_block() { return kBinsMixed; }
 
 hipError_t launch_iota(uint32_t* idx, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -428,13 +598,50 @@ hipError_t launch_accum_f64(const PointConsts& pc, const GridView& gv, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_accum_mixed(const PointConsts& pc, const GridView& gv, const double* nu_sorted, DerivedF64 d,
-                              const int2* ranges, int nblocks, double* intens, hipStream_t s) {
-  if (nblocks <= 0) return hipSuccess;
-  if (pc.profile != 2) return hipErrorInvalidValue;
-  k_accum_voigt_mixed<<<nblocks, kBlock, 0, s>>>(gv, nu_sorted, d.S, d.g, d.y, d.ib, d.ie, ranges, nblocks, intens,
-                                                 1e-6);
+hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double* nu, int n, int* ib, int* ie,
+                          unsigned long long* evals, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_windows<<<(n + 255) / 256, 256, 0, s>>>(wa, gv, nu, n, ib, ie, evals);
   return hipGetLastError();
+}
+
+hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, MixedLineData out,
+                             hipStream_t s) {
+  if (lv.n <= 0 || npts <= 0) return hipSuccess;
+  k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 1 << 20);
+  k_fill<<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+int mixed_bins_per_block(int R) { return R == 4 ? MixedShape<4>::kBins : MixedShape<8>::kBins; }
+
+hipError_t launch_accum_mixed(const MixedPlan& p, int R, hipStream_t s) {
+  if (p.nitems <= 0) return hipSuccess;
+  k_reset_counter<<<1, 1, 0, s>>>(p.counter);
+  if (R == 4)
+    k_accum_voigt_mixed<4><<<p.grid, kBlock, 0, s>>>(p);
+  else
+    k_accum_voigt_mixed<8><<<p.grid, kBlock, 0, s>>>(p);
+  hipError_t e = hipGetLastError();
+  if (e || p.nseg == 1) return e;
+  const int bpb = mixed_bins_per_block(R);
+  const long long n = (long long)p.nblocks * bpb * p.ntasks;
+  k_reduce_segments<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, bpb);
+  return hipGetLastError();
+}
+
+int mixed_occupancy(int R) {
+  int n = 0;
+  hipError_t e = R == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<4>, kBlock, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<8>, kBlock, 0);
+  if (e != hipSuccess) n = 1;
+  return n > 0 ? n : 1;
 }
 
 }  // namespace gx

@@ -1,0 +1,469 @@
+// gx_table.cpp -- batched (T,P) opacity tables over one resident line list.
+//
+// The reference computes one (T,P) point per process: InitializeConstants is
+// called once (VoigtProfile.cpp:26) and every cross-section re-reads and
+// re-uploads its lines.  Here the sorted line list stays in HBM (gx_lines)
+// and a table of points is planned once (gx_table_create):
+//
+//   fill(out = 0) -> windows per cutoff group -> block ranges per group
+//   -> per-point intensities of all points (one 2-D launch)
+//   -> ONE persistent accumulation launch whose work queue spans every
+//      (point, bin block, line segment) item of the table
+//
+// and that sequence is captured into a hipGraph on first use and replayed by
+// gx_table_run.  Points are admitted by free HBM: when the per-point line
+// data of all points does not fit, the table runs in point batches, each its
+// own persistent launch inside the same graph.  An autotuner times the
+// launch shape (bins per lane, work split) on the first points once.
+//
+// The FP64 precision of a table is the plain per-point path: one intensity,
+// ranges and accumulation launch per point, the denominator of the speedup
+// the mixed path is measured against.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gx.h"
+#include "gx_common.h"
+
+using namespace gx;
+
+struct gx_lines {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  LineStore ls;
+  bool voigt = false;
+};
+
+namespace {
+
+struct Group {                // points that share one cutoff (hence windows and block ranges)
+  double cutoff = 0;
+  std::vector<int> points;
+  int first_const = 0;        // index of points[0] in the grouped constants table
+  int* ib = nullptr;
+  int* ie = nullptr;
+  Ranges rng;                 // mixed block ranges
+  Ranges rng64;               // FP64 block ranges
+  int nblocks = 0;
+  int nblocks64 = 0;
+  unsigned long long* evals = nullptr;   // kEvalSlots
+};
+
+struct Launch {               // one persistent accumulation launch (a batch of a group)
+  int group = 0;
+  int first = 0;              // index into the group's point list
+  int count = 0;
+  MixedRunner run;
+};
+
+// autotune cache: (bins, lines, points per launch) -> (R, items per slot)
+std::map<std::tuple<int, size_t, int>, std::pair<int, int>> g_tuned;
+
+}  // namespace
+
+struct gx_table {
+  gx_lines* L = nullptr;
+  gx_grid grid{};
+  double dfreq = 0;
+  int M = 0;
+  std::vector<gx_point> pts;
+  gx_table_opts o{};
+  std::vector<PointConsts> pcs;
+  double* d_freq = nullptr;
+  double* d_out = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  std::vector<Group> groups;
+  std::vector<Launch> launches;
+  Derived der;                // mixed line data for `batch` points, all points' constants
+  Derived der64;              // FP64 path
+  int batch = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_ran = false;
+  float eager_intensity_ms = 0, eager_accumulate_ms = 0;
+  int R = 4, ips = 4;
+};
+
+namespace {
+
+bool mixed_table(const gx_table* t) { return t->o.precision == GX_PREC_MIXED && t->o.profile == GX_VOIGT; }
+
+GridView grid_view(const gx_table* t) {
+  return GridView{t->d_freq, t->M, 0, t->M, t->grid.start_nu, t->dfreq};
+}
+
+LinesView lines_view(const gx_table* t) {
+  const LineStore& ls = t->L->ls;
+  return LinesView{ls.nu, ls.elow, ls.aif, ls.gns, ls.gam, ls.nexp, (int)ls.n};
+}
+
+// Everything one run enqueues; graph-capturable (kernels only, no memset, no host sync).
+// timed: record ev[0..2] around the phases (only outside capture).
+int enqueue(gx_table* t, bool timed) {
+  hipStream_t s = t->stream;
+  hipError_t e;
+  const GridView gv = grid_view(t);
+  const LinesView lv = lines_view(t);
+  if (timed && (e = hipEventRecord(t->ev[0], s))) return hip_fail(e, "event");
+  if ((e = launch_fill(t->d_out, (size_t)t->M * t->pts.size(), 0.0, s))) return hip_fail(e, "fill");
+  for (Group& g : t->groups) {
+    if ((e = launch_fill(reinterpret_cast<double*>(g.evals), kEvalSlots, 0.0, s))) return hip_fail(e, "fill");
+    const WindowArgs wa{g.cutoff, t->o.convention, 1};
+    if ((e = launch_windows(wa, gv, lv.nu, lv.n, g.ib, g.ie, g.evals, s))) return hip_fail(e, "windows");
+  }
+  const bool mixed = mixed_table(t);
+  if (mixed) {
+    for (Group& g : t->groups) {
+      const double cut = g.cutoff;
+      if ((e = launch_block_ranges(gv, lv.nu, lv.n, cut, mixed_bins_per_block(t->R), g.nblocks, g.rng.r, s)))
+        return hip_fail(e, "ranges");
+    }
+    bool first = true;
+    for (Launch& l : t->launches) {
+      const Group& g = t->groups[l.group];
+      // points of one launch are contiguous in the constants table (grouped at create)
+      const PointConsts* pc0 = t->der.d_pcs + g.first_const + l.first;
+      if ((e = launch_prep_mixed(pc0, l.count, lv, t->der.m, s))) return hip_fail(e, "prep");
+      if (first && timed && (e = hipEventRecord(t->ev[1], s))) return hip_fail(e, "event");
+      first = false;
+      int rc = l.run.launch(gv, lv.nu, g.rng.r, g.ib, g.ie, s);
+      if (rc) return rc;
+    }
+  } else {
+    if (timed && (e = hipEventRecord(t->ev[1], s))) return hip_fail(e, "event");
+    for (Group& g : t->groups) {
+      if ((e = launch_block_ranges(gv, lv.nu, lv.n, g.cutoff, kBinsF64, g.nblocks64, g.rng64.r, s)))
+        return hip_fail(e, "ranges");
+      for (int p : g.points) {
+        if ((e = launch_line_prep_f64(t->pcs[p], gv, lv, t->der64.d, nullptr, s))) return hip_fail(e, "prep");
+        if ((e = launch_accum_f64(t->pcs[p], gv, lv.nu, t->der64.d, g.rng64.r, g.nblocks64, t->d_out + (size_t)p * t->M,
+                                  s)))
+          return hip_fail(e, "accumulate");
+      }
+    }
+  }
+  if (timed && (e = hipEventRecord(t->ev[2], s))) return hip_fail(e, "event");
+  return GX_OK;
+}
+
+int prepare_launches(gx_table* t) {
+  for (Launch& l : t->launches) {
+    const Group& g = t->groups[l.group];
+    l.run.R = t->R;
+    l.run.items_per_slot = t->ips;
+    std::vector<MixedTask> tasks(l.count);
+    for (int k = 0; k < l.count; ++k) {
+      const int p = g.points[l.first + k];
+      const size_t off = (size_t)k * t->der.m.stride;
+      tasks[k] = MixedTask{t->der.m.amp + off, t->der.m.y + off, t->pcs[p].dpwcoef, t->d_out + (size_t)p * t->M};
+    }
+    const int nb = (t->M + mixed_bins_per_block(t->R) - 1) / mixed_bins_per_block(t->R);
+    int rc = l.run.prepare(tasks.data(), l.count, nb, t->stream);
+    if (rc) return rc;
+  }
+  for (Group& g : t->groups) {
+    g.nblocks = (t->M + mixed_bins_per_block(t->R) - 1) / mixed_bins_per_block(t->R);
+    int rc = g.rng.reserve(g.nblocks);
+    if (rc) return rc;
+  }
+  return GX_OK;
+}
+
+// Time each launch shape eagerly on this table and keep the fastest.
+int autotune(gx_table* t) {
+  const auto key = std::make_tuple(t->M, t->L->ls.n, t->launches.empty() ? 0 : t->launches[0].count);
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) {
+    t->R = it->second.first;
+    t->ips = it->second.second;
+    return prepare_launches(t);
+  }
+  float best = 1e30f;
+  int bR = 4, bI = 4;
+  int last_nseg = -1, last_R = -1;
+  for (int R : {8, 4}) {
+    for (int ips : {4, 8, 2}) {
+      t->R = R;
+      t->ips = ips;
+      int rc = prepare_launches(t);
+      if (rc) return rc;
+      const int nseg = t->launches.empty() ? 1 : t->launches[0].run.plan.nseg;
+      if (R == last_R && nseg == last_nseg) continue;   // same launch shape as a timed one
+      last_R = R;
+      last_nseg = nseg;
+      if ((rc = enqueue(t, true))) return rc;
+      hipError_t e;
+      if ((e = hipStreamSynchronize(t->stream))) return hip_fail(e, "autotune");
+      float ms = 0;
+      if ((e = hipEventElapsedTime(&ms, t->ev[1], t->ev[2]))) return hip_fail(e, "autotune events");
+      if (ms < best) {
+        best = ms;
+        bR = R;
+        bI = ips;
+      }
+    }
+  }
+  t->R = bR;
+  t->ips = bI;
+  g_tuned[key] = {bR, bI};
+  return prepare_launches(t);
+}
+
+void free_table(gx_table* t) {
+  if (!t) return;
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  if (t->exec) (void)hipGraphExecDestroy(t->exec);
+  if (t->graph) (void)hipGraphDestroy(t->graph);
+  for (Group& g : t->groups) {
+    (void)hipFree(g.ib);
+    (void)hipFree(g.ie);
+    (void)hipFree(g.evals);
+    g.rng.release();
+    g.rng64.release();
+  }
+  for (Launch& l : t->launches) l.run.release();
+  t->der.release();
+  t->der64.release();
+  (void)hipFree(t->d_freq);
+  (void)hipFree(t->d_out);
+  for (auto& ev : t->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+  delete t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_lines_create(int device, size_t n, const double* energies, const double* nu, const double* aif, const int* gns,
+                    const double* gamma_ref, const double* nexp, gx_lines** out) {
+  if (!out) return fail(GX_INVALID_ARG, "lines_create: out is NULL");
+  *out = nullptr;
+  if (n == 0 || !energies || !nu || !aif || !gns) return fail(GX_INVALID_ARG, "lines_create: empty or NULL arrays");
+  if ((gamma_ref == nullptr) != (nexp == nullptr)) return fail(GX_INVALID_ARG, "lines_create: gamma_ref and nexp go together");
+  if (n > (size_t)INT32_MAX) return fail(GX_INVALID_ARG, "lines_create: more than 2^31-1 lines");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GX_NO_DEVICE, "lines_create: no HIP device");
+  if (device < 0 || device >= ndev) return fail(GX_NO_DEVICE, "lines_create: device index out of range");
+  hipError_t e;
+  if ((e = hipSetDevice(device))) return hip_fail(e, "hipSetDevice");
+  gx_lines* L = new gx_lines();
+  L->device = device;
+  L->voigt = gamma_ref != nullptr;
+  if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking))) { delete L; return hip_fail(e, "stream"); }
+  int rc = L->ls.upload(n, energies, nu, aif, gns, gamma_ref, nexp, L->stream);
+  if (rc) { gx_lines_destroy(L); return rc; }
+  *out = L;
+  return GX_OK;
+}
+
+int gx_lines_destroy(gx_lines* L) {
+  if (!L) return GX_OK;
+  (void)hipSetDevice(L->device);
+  if (L->stream) (void)hipStreamSynchronize(L->stream);
+  L->ls.release();
+  if (L->stream) (void)hipStreamDestroy(L->stream);
+  delete L;
+  return GX_OK;
+}
+
+size_t gx_lines_count(const gx_lines* L) { return L ? L->ls.n : 0; }
+
+int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int npts, const gx_table_opts* o,
+                    gx_table** out) {
+  if (!out) return fail(GX_INVALID_ARG, "table_create: out is NULL");
+  *out = nullptr;
+  if (!L || !grid || !pts || !o || npts <= 0) return fail(GX_INVALID_ARG, "table_create: bad arguments");
+  if (grid->npoints <= 0 || !(grid->end_nu > grid->start_nu)) return fail(GX_INVALID_ARG, "table_create: bad grid");
+  if (o->profile != GX_VOIGT && o->profile != GX_DOPPLER) return fail(GX_INVALID_ARG, "table_create: bad profile");
+  if (o->profile == GX_VOIGT && !L->voigt) return fail(GX_INVALID_ARG, "table_create: Voigt needs gamma_ref and nexp");
+  if (!(o->mean_mass > 0) || !(o->ref_temp > 0)) return fail(GX_INVALID_ARG, "table_create: mean_mass and ref_temp must be > 0");
+  for (int p = 0; p < npts; ++p)
+    if (!(pts[p].temperature > 0) || !(pts[p].partition > 0) || !std::isfinite(pts[p].pressure))
+      return fail(GX_INVALID_ARG, "table_create: bad point");
+  hipError_t e;
+  if ((e = hipSetDevice(L->device))) return hip_fail(e, "hipSetDevice");
+  gx_table* t = new gx_table();
+  t->L = L;
+  t->grid = *grid;
+  t->M = grid->npoints;
+  t->dfreq = (grid->end_nu - grid->start_nu) / double(grid->npoints);   // BaseProfile.cpp:10
+  t->pts.assign(pts, pts + npts);
+  t->o = *o;
+  auto bail = [&](int rc) { free_table(t); return rc; };
+  if ((e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking))) return bail(hip_fail(e, "stream"));
+  for (auto& ev : t->ev)
+    if ((e = hipEventCreate(&ev))) return bail(hip_fail(e, "event"));
+  // grid nodes, BaseProfile.cpp:17-21
+  std::vector<double> f(t->M);
+  for (int i = 0; i < t->M; i++) f[i] = grid->start_nu + double(i) * t->dfreq;
+  if ((e = hipMalloc(&t->d_freq, sizeof(double) * t->M))) return bail(hip_fail(e, "hipMalloc(grid)"));
+  if ((e = hipMemcpy(t->d_freq, f.data(), sizeof(double) * t->M, hipMemcpyHostToDevice))) return bail(hip_fail(e, "grid upload"));
+  if ((e = hipMalloc(&t->d_out, sizeof(double) * (size_t)t->M * npts))) return bail(hip_fail(e, "hipMalloc(table grids)"));
+
+  // group points by cutoff; constants in group order so each launch's points are contiguous
+  std::vector<int> order;
+  std::map<double, int> gidx;
+  for (int p = 0; p < npts; ++p) {
+    const double c = reference_cutoff(o->profile, pts[p].pressure, o->cutoff);
+    auto it = gidx.find(c);
+    if (it == gidx.end()) {
+      gidx[c] = (int)t->groups.size();
+      t->groups.emplace_back();
+      t->groups.back().cutoff = c;
+    }
+    t->groups[gidx[c]].points.push_back(p);
+  }
+  const size_t n = L->ls.n;
+  t->pcs.resize(npts);
+  std::vector<PointConsts> grouped;   // constants in group order: a launch's points are contiguous
+  for (Group& g : t->groups) {
+    g.first_const = (int)grouped.size();
+    for (int p : g.points) {
+      gx_consts c{};
+      c.temperature = pts[p].temperature;
+      c.partition = pts[p].partition;
+      c.dfreq = t->dfreq;
+      c.mean_mass = o->mean_mass;
+      c.pressure = pts[p].pressure;
+      c.ref_temp = o->ref_temp;
+      c.cutoff = g.cutoff;
+      c.doppler_widths = o->doppler_widths;
+      c.convention = o->convention;
+      c.precision = o->precision;
+      t->pcs[p] = make_point_consts(o->profile, c, pts[p].pressure);
+      grouped.push_back(t->pcs[p]);
+    }
+  }
+  for (Group& g : t->groups) {
+    if ((e = hipMalloc(&g.ib, 4 * std::max<size_t>(n, 1))) || (e = hipMalloc(&g.ie, 4 * std::max<size_t>(n, 1))) ||
+        (e = hipMalloc(&g.evals, sizeof(unsigned long long) * kEvalSlots)))
+      return bail(hip_fail(e, "hipMalloc(windows)"));
+    g.nblocks64 = (t->M + kBinsF64 - 1) / kBinsF64;
+    int rc = g.rng64.reserve(g.nblocks64);
+    if (rc) return bail(rc);
+  }
+  if (mixed_table(t)) {
+    // admission: per-point FP32 line data of as many points as free HBM allows
+    size_t free_b = 0, total_b = 0;
+    if ((e = hipMemGetInfo(&free_b, &total_b))) return bail(hip_fail(e, "hipMemGetInfo"));
+    const size_t headroom = size_t(2) << 30;
+    const size_t per_point = 12 * std::max<size_t>(n, 1) + (size_t(64) << 10);
+    const size_t avail = free_b > headroom ? free_b - headroom : 0;
+    t->batch = (int)std::min<size_t>((size_t)npts, avail / per_point);
+    if (t->batch < 1) return bail(fail(GX_OUT_OF_MEMORY, "table_create: not enough free HBM for one point"));
+    int rc = t->der.reserve_mixed(n, t->batch);
+    if (rc) return bail(rc);
+    if ((rc = t->der.set_points(grouped.data(), npts, t->stream))) return bail(rc);
+    // launches: batches of consecutive points inside each group
+    for (int gi = 0; gi < (int)t->groups.size(); ++gi) {
+      const int np = (int)t->groups[gi].points.size();
+      for (int f0 = 0; f0 < np; f0 += t->batch) {
+        t->launches.emplace_back();
+        t->launches.back().group = gi;
+        t->launches.back().first = f0;
+        t->launches.back().count = std::min(t->batch, np - f0);
+      }
+    }
+    if ((rc = o->autotune ? autotune(t) : prepare_launches(t))) return bail(rc);
+  } else {
+    int rc = t->der64.reserve(std::max<size_t>(n, 1));
+    if (rc) return bail(rc);
+  }
+  *out = t;
+  return GX_OK;
+}
+
+int gx_table_run(gx_table* t) {
+  if (!t) return fail(GX_INVALID_ARG, "table_run: NULL table");
+  hipError_t e;
+  if ((e = hipSetDevice(t->L->device))) return hip_fail(e, "hipSetDevice");
+  if (!t->o.use_graph) {
+    t->graph_ran = false;
+    return enqueue(t, true);
+  }
+  if (!t->exec) {
+    // one eager run first: its events give the kernel split the graph run cannot record
+    int rc = enqueue(t, true);
+    if (rc) return rc;
+    if ((e = hipStreamSynchronize(t->stream))) return hip_fail(e, "table_run: eager");
+    if ((e = hipEventElapsedTime(&t->eager_intensity_ms, t->ev[0], t->ev[1])) ||
+        (e = hipEventElapsedTime(&t->eager_accumulate_ms, t->ev[1], t->ev[2])))
+      return hip_fail(e, "table_run: events");
+    if ((e = hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal))) return hip_fail(e, "begin capture");
+    rc = enqueue(t, false);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(t->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e2) return hip_fail(e2, "end capture");
+    t->graph = g;
+    if ((e = hipGraphInstantiate(&t->exec, t->graph, nullptr, nullptr, 0))) return hip_fail(e, "graph instantiate");
+  }
+  if ((e = hipEventRecord(t->ev[0], t->stream))) return hip_fail(e, "event");
+  if ((e = hipGraphLaunch(t->exec, t->stream))) return hip_fail(e, "graph launch");
+  if ((e = hipEventRecord(t->ev[2], t->stream))) return hip_fail(e, "event");
+  t->graph_ran = true;
+  return GX_OK;
+}
+
+int gx_table_wait(gx_table* t, gx_timing* timing) {
+  if (!t) return fail(GX_INVALID_ARG, "table_wait: NULL table");
+  hipError_t e;
+  if ((e = hipStreamSynchronize(t->stream))) return hip_fail(e, "table_wait");
+  if (!timing) return GX_OK;
+  float total = 0, a = 0, b = 0;
+  if ((e = hipEventElapsedTime(&total, t->ev[0], t->ev[2]))) return hip_fail(e, "table_wait: events");
+  if (t->graph_ran) {
+    a = t->eager_intensity_ms;
+    b = total - a;
+  } else {
+    if ((e = hipEventElapsedTime(&a, t->ev[0], t->ev[1])) || (e = hipEventElapsedTime(&b, t->ev[1], t->ev[2])))
+      return hip_fail(e, "table_wait: events");
+  }
+  long long evals = 0;
+  for (Group& g : t->groups) {
+    unsigned long long slots[kEvalSlots];
+    if ((e = hipMemcpy(slots, g.evals, sizeof(slots), hipMemcpyDeviceToHost))) return hip_fail(e, "table_wait: copy");
+    unsigned long long s = 0;
+    for (auto v : slots) s += v;
+    evals += (long long)s * (long long)g.points.size();
+  }
+  timing->intensity_ms = a;
+  timing->accumulate_ms = b;
+  timing->total_ms = total;
+  timing->evals = evals;
+  timing->lines = (long long)t->L->ls.n;
+  return GX_OK;
+}
+
+int gx_table_read(gx_table* t, int point, double* out, int n) {
+  if (!t || !out || point < 0 || point >= (int)t->pts.size() || n <= 0 || n > t->M)
+    return fail(GX_INVALID_ARG, "table_read: bad arguments");
+  hipError_t e;
+  if ((e = hipStreamSynchronize(t->stream))) return hip_fail(e, "table_read: sync");
+  if ((e = hipMemcpy(out, t->d_out + (size_t)point * t->M, sizeof(double) * n, hipMemcpyDeviceToHost)))
+    return hip_fail(e, "table_read: copy");
+  return GX_OK;
+}
+
+int gx_table_shape(gx_table* t, int* bins_per_lane, int* segments) {
+  if (!t) return fail(GX_INVALID_ARG, "table_shape: NULL table");
+  if (bins_per_lane) *bins_per_lane = mixed_table(t) ? t->R : 1;
+  if (segments) *segments = (mixed_table(t) && !t->launches.empty()) ? t->launches[0].run.plan.nseg : 1;
+  return GX_OK;
+}
+
+int gx_table_destroy(gx_table* t) {
+  free_table(t);
+  return GX_OK;
+}
+
+}  // extern "C"

@@ -62,10 +62,14 @@ typedef struct gx_consts {
     int precision;          /* GX_PREC_* */
 } gx_consts;
 
-/* Kernel times of the last gx_execute, from HIP events on the context's stream. */
+/* Kernel times of the last gx_execute / table run, from HIP events on the
+ * stream the kernels run on.  A graph replay is timed as a whole (total_ms);
+ * its intensity_ms is that of the eager run made before capture and
+ * accumulate_ms the remainder. */
 typedef struct gx_timing {
-    float intensity_ms;     /* line-intensity kernel */
-    float accumulate_ms;    /* bin-parallel profile accumulation kernel */
+    float intensity_ms;     /* line-intensity (and window) kernels */
+    float accumulate_ms;    /* bin-parallel profile accumulation kernels */
+    float total_ms;         /* whole call */
     long long evals;        /* line-bin evaluations sum_i (ie_i - ib_i) of that call */
     long long lines;        /* lines processed */
 } gx_timing;
@@ -117,6 +121,62 @@ int gx_synchronize(gx_ctx *ctx);
 
 /* Kernel timing of the last gx_execute (synchronises). */
 int gx_last_timing(gx_ctx *ctx, gx_timing *out);
+
+/* ------------------------------------------------------------------------
+ * Batched (T,P) opacity tables over one resident line list.  No reference
+ * counterpart: the reference computes one (T,P) point per process
+ * (VoigtProfile.cpp:26).  Each grid of a table equals what the manager path
+ * above computes for that point alone.
+ * ------------------------------------------------------------------------ */
+typedef struct gx_lines gx_lines;
+typedef struct gx_table gx_table;
+
+/* Grid rule of BaseProfile.cpp:7-21: dfreq = (end-start)/N, f_i = start + i*dfreq. */
+typedef struct gx_grid {
+    double start_nu;
+    double end_nu;
+    int npoints;
+} gx_grid;
+
+typedef struct gx_point {
+    double temperature;     /* K */
+    double pressure;        /* atm (multiplies gamma_ref) */
+    double partition;       /* Q(T) */
+} gx_point;
+
+typedef struct gx_table_opts {
+    int profile;            /* GX_VOIGT, or GX_DOPPLER (FP64 only) */
+    int precision;          /* GX_PREC_* */
+    int convention;         /* GX_CONV_* */
+    double mean_mass;       /* amu */
+    double ref_temp;        /* K */
+    double cutoff;          /* > 0: fixed; <= 0: reference rule per point */
+    double doppler_widths;  /* Doppler: > 0 per-line window k*alpha + dfreq */
+    int use_graph;          /* capture the launch sequence once and replay it */
+    int autotune;           /* time launch shapes on this table once, keep the fastest */
+} gx_table_opts;
+
+/* Upload a line list once (sorted by nu on the device).  gamma_ref is per
+ * atm (NULL with nexp NULL: Doppler-only list). */
+int gx_lines_create(int device, size_t n, const double *energies, const double *nu, const double *aif,
+                    const int *gns, const double *gamma_ref, const double *nexp, gx_lines **out);
+int gx_lines_destroy(gx_lines *lines);
+size_t gx_lines_count(const gx_lines *lines);
+
+/* Plan a table of npts points on one grid: allocates per-point line data and
+ * the npts output grids (admission by free HBM: GX_OUT_OF_MEMORY when the
+ * table cannot fit even one point at a time). */
+int gx_table_create(gx_lines *lines, const gx_grid *grid, const gx_point *pts, int npts,
+                    const gx_table_opts *opts, gx_table **out);
+/* Recompute every grid of the table (asynchronous). */
+int gx_table_run(gx_table *table);
+/* Wait for the last run; timing (may be NULL) gets its kernel times and evaluation count. */
+int gx_table_wait(gx_table *table, gx_timing *timing);
+/* Copy grid `point` (n bins) to the host. */
+int gx_table_read(gx_table *table, int point, double *out, int n);
+/* Launch shape in use: bins per lane and line segments per block. */
+int gx_table_shape(gx_table *table, int *bins_per_lane, int *segments);
+int gx_table_destroy(gx_table *table);
 
 /* Message of the last failing call on this host thread. */
 const char *gx_last_error(void);

@@ -176,3 +176,23 @@ def test_edge_cases():
         with pytest.raises(capi.GxError):
             m.ExecuteCrossSection(10, L.n + 1, 0)                      # more lines than uploaded
         assert m.ReadyForWork()
+
+
+def test_voigt_mixed_tiny_nu_lines():
+    """Lines at nu ~ 1e-5..30 cm^-1 have 1/alpha up to ~1e10: their far wings must not
+    overflow the FP32 region-I evaluation (they dominate the bins near nu = 0)."""
+    rng = np.random.default_rng(17)
+    n = 3000
+    nu = 10 ** rng.uniform(-5, np.log10(30.0), n)
+    L = make_lines(5, 2000, 20000.0, n, 12000.0, 17.03)
+    L.nu = np.ascontiguousarray(nu)
+    L.elow = np.ascontiguousarray(rng.uniform(0, 3000, n))
+    T, P, Q = 1500.0, 0.1, 1234.0
+    freq, d = make_grid(0.0, 30.0, 6000)
+    gam = L.gamma_at(P)
+    out, _ = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED, cutoff=25.0)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=25.0)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(out, ref)
+    print(f"  tiny-nu mixed max rel err {err:.3e}", flush=True)
+    assert err <= MIXED_TOL
