@@ -176,43 +176,54 @@ int prepare_launches(gx_table* t) {
   return GX_OK;
 }
 
-// Time each launch shape eagerly on this table and keep the fastest.
+// Time each launch shape on the first (up to 4) points of the first group and keep
+// the fastest; the choice is cached per (bins, lines, points per launch).
 int autotune(gx_table* t) {
   const auto key = std::make_tuple(t->M, t->L->ls.n, t->launches.empty() ? 0 : t->launches[0].count);
   auto it = g_tuned.find(key);
+  if (it == g_tuned.end() && !t->launches.empty()) {
+    std::vector<Launch> saved;
+    saved.swap(t->launches);
+    t->launches.emplace_back();
+    t->launches[0].group = saved[0].group;
+    t->launches[0].first = 0;
+    t->launches[0].count = std::min(4, saved[0].count);
+    float best = 1e30f;
+    int bR = 4, bI = 4;
+    int last_nseg = -1, last_R = -1;
+    int rc = GX_OK;
+    for (int R : {4, 8}) {
+      for (int ips : {4, 8, 2}) {
+        t->R = R;
+        t->ips = ips;
+        if ((rc = prepare_launches(t))) break;
+        const int nseg = t->launches[0].run.plan.nseg;
+        if (R == last_R && nseg == last_nseg) continue;   // same launch shape as a timed one
+        last_R = R;
+        last_nseg = nseg;
+        if ((rc = enqueue(t, true))) break;
+        hipError_t e;
+        if ((e = hipStreamSynchronize(t->stream))) { rc = hip_fail(e, "autotune"); break; }
+        float ms = 0;
+        if ((e = hipEventElapsedTime(&ms, t->ev[1], t->ev[2]))) { rc = hip_fail(e, "autotune events"); break; }
+        if (ms < best) {
+          best = ms;
+          bR = R;
+          bI = ips;
+        }
+      }
+      if (rc) break;
+    }
+    t->launches[0].run.release();
+    t->launches.swap(saved);
+    if (rc) return rc;
+    g_tuned[key] = {bR, bI};
+    it = g_tuned.find(key);
+  }
   if (it != g_tuned.end()) {
     t->R = it->second.first;
     t->ips = it->second.second;
-    return prepare_launches(t);
   }
-  float best = 1e30f;
-  int bR = 4, bI = 4;
-  int last_nseg = -1, last_R = -1;
-  for (int R : {8, 4}) {
-    for (int ips : {4, 8, 2}) {
-      t->R = R;
-      t->ips = ips;
-      int rc = prepare_launches(t);
-      if (rc) return rc;
-      const int nseg = t->launches.empty() ? 1 : t->launches[0].run.plan.nseg;
-      if (R == last_R && nseg == last_nseg) continue;   // same launch shape as a timed one
-      last_R = R;
-      last_nseg = nseg;
-      if ((rc = enqueue(t, true))) return rc;
-      hipError_t e;
-      if ((e = hipStreamSynchronize(t->stream))) return hip_fail(e, "autotune");
-      float ms = 0;
-      if ((e = hipEventElapsedTime(&ms, t->ev[1], t->ev[2]))) return hip_fail(e, "autotune events");
-      if (ms < best) {
-        best = ms;
-        bR = R;
-        bI = ips;
-      }
-    }
-  }
-  t->R = bR;
-  t->ips = bI;
-  g_tuned[key] = {bR, bI};
   return prepare_launches(t);
 }
 
