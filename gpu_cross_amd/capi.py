@@ -60,6 +60,29 @@ class TableOpts(ctypes.Structure):
                 ("doppler_widths", ctypes.c_double), ("use_graph", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("temperature", ctypes.c_double), ("pressure", ctypes.c_double), ("partition", ctypes.c_double),
+                ("profile", ctypes.c_int), ("precision", ctypes.c_int), ("start_nu", ctypes.c_double),
+                ("end_nu", ctypes.c_double), ("npoints", ctypes.c_int), ("cutoff", ctypes.c_double),
+                ("doppler_widths", ctypes.c_double)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("mean_mass", ctypes.c_double), ("ref_temp", ctypes.c_double), ("convention", ctypes.c_int),
+                ("max_inflight", ctypes.c_int), ("max_batch", ctypes.c_int), ("hbm_budget", ctypes.c_size_t)]
+
+
+class ServerSummary(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("cancelled", ctypes.c_longlong),
+                ("failed", ctypes.c_longlong), ("batches", ctypes.c_longlong), ("batched_requests", ctypes.c_longlong),
+                ("evals", ctypes.c_longlong), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("peak_bytes", ctypes.c_size_t)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
     return LIB_PATH
@@ -113,6 +136,14 @@ def load():
     _sig(L, "gx_table_read", ctypes.c_int, [P, ctypes.c_int, _dp, ctypes.c_int])
     _sig(L, "gx_table_shape", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)])
     _sig(L, "gx_table_destroy", ctypes.c_int, [P])
+    _sig(L, "gx_server_create", ctypes.c_int, [P, ctypes.POINTER(ServerOpts), ctypes.POINTER(P)])
+    _sig(L, "gx_server_submit", ctypes.c_int, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)])
+    _sig(L, "gx_server_poll", ctypes.c_int, [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)])
+    _sig(L, "gx_server_wait", ctypes.c_int, [P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)])
+    _sig(L, "gx_server_result", ctypes.c_int, [P, ctypes.c_uint64, _dp, ctypes.c_int, ctypes.POINTER(Timing)])
+    _sig(L, "gx_server_cancel", ctypes.c_int, [P, ctypes.c_uint64])
+    _sig(L, "gx_server_stats", ctypes.c_int, [P, ctypes.POINTER(ServerSummary)])
+    _sig(L, "gx_server_destroy", ctypes.c_int, [P])
     _sig(L, "gx_last_error", ctypes.c_char_p, [])
     _sig(L, "gx_version", ctypes.c_char_p, [])
     _lib = L

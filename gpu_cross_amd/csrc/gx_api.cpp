@@ -320,6 +320,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStr
   plan.counter = counter;
   plan.nitems = (int)nitems;
   plan.grid = (int)std::min<long long>(nitems, slots);
+  if (max_grid > 0) plan.grid = std::min(plan.grid, max_grid);
   return GX_OK;
 }
 

@@ -79,6 +79,7 @@ struct MixedRunner {
   int R = 4;                   // bins per lane (4 or 8)
   int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
   int forced_nseg = 0;         // > 0: fixed segments per block
+  int max_grid = 0;            // > 0: cap on persistent workgroups (device shared with other launches)
   std::vector<MixedTask> h_tasks;
   MixedPlan plan{};
   int init();
@@ -98,3 +99,12 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
               hipStream_t s, hipEvent_t* ev);
 
 }  // namespace gx
+
+// Resident line list (gx_lines_* in include/gx.h), shared by tables and the server.
+struct gx_lines {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  gx::LineStore ls;
+  bool voigt = false;
+  std::vector<double> h_nu;   // host copy of the sorted centres, for slicing by frequency
+};

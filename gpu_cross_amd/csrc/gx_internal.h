@@ -119,6 +119,7 @@ hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double
 hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, MixedLineData out,
                              hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
+hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s);
 // R = bins per lane (4 or 8); a workgroup covers mixed_bins_per_block(R) bins
 hipError_t launch_accum_mixed(const MixedPlan& plan, int R, hipStream_t s);
 int mixed_occupancy(int R);   // resident workgroups per CU of the mixed kernel

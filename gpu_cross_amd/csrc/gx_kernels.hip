@@ -30,6 +30,13 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// grid nodes f_i = start + i*dfreq exactly as BaseProfile.cpp:19 forms them (no FMA)
+__global__ void k_grid(double* __restrict__ f, int n, double start, double dfreq) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = start + (double)i * dfreq;
+}
+
 __global__ void k_iota(uint32_t* idx, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) idx[i] = (uint32_t)i;
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
             if (jj >= I.x && jj < I.y) {
               if (jj >= I.z && jj < I.w) {
                 const double x = fabs(s_nu[k] - gv.freq[jj]) * s_g[k];
-                part[q] += s_amp[k] * (float)w4_inner(x, s_y[k]);
+                part[q] += s_amp[k] * w4_core_mixed(x, s_y[k]);
               } else {
                 part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
               }
@@ -609,6 +616,12 @@ hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView
                              hipStream_t s) {
   if (lv.n <= 0 || npts <= 0) return hipSuccess;
   k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_grid<<<(n + 255) / 256, 256, 0, s>>>(f, n, start, dfreq);
   return hipGetLastError();
 }
 

@@ -1,0 +1,582 @@
+// gx_server.cpp -- continuous batching of cross-section requests on one GPU.
+//
+// One scheduler thread owns every device call.  Clients submit requests
+// (T, P, Q, profile, grid) from any thread; the scheduler
+//   1. retires finished batches (event query, never a blocking wait on a
+//      running batch), copies their grids to host memory and frees budget;
+//   2. admits queued requests, oldest first, while an in-flight slot is free
+//      and the batch fits the HBM budget, pulling every compatible queued
+//      request (same grid, profile, precision, cutoff) into the same batch;
+//   3. launches the batch on the slot's own stream, so batches overlap on
+//      the device and new requests start while others are still running.
+// A request only touches the slice of the resident, nu-sorted line list
+// that can reach its grid.  Mixed-precision batches are one persistent
+// launch over all requests of the batch (the table kernels); the work split
+// per request depends on the request's own grid only, so its result is the
+// same alone and in any mix.  FP64 and Doppler requests run the plain
+// per-point kernels.  Slot buffers grow and are reused (no per-batch
+// hipMalloc/hipFree on the critical path).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gx.h"
+#include "gx_common.h"
+
+using namespace gx;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Req {
+  uint64_t id = 0;
+  gx_request r{};
+  int state = GX_REQ_QUEUED;
+  bool cancel = false;
+  Clock::time_point t_submit, t_done;
+  std::vector<double> host;
+  long long evals = 0;
+  float kernel_ms = 0;
+  std::string error;
+};
+
+// Device buffers of one in-flight slot; they only grow.
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_done = nullptr;
+  bool busy = false;
+  std::vector<Req*> reqs;
+  std::vector<PointConsts> h_pcs;
+  double* freq = nullptr;
+  size_t freq_cap = 0;
+  double* out = nullptr;
+  size_t out_cap = 0;
+  int* ib = nullptr;
+  int* ie = nullptr;
+  size_t win_cap = 0;
+  unsigned long long* evals = nullptr;
+  Derived der;
+  Ranges rng;
+  MixedRunner run;
+  Clock::time_point t_launch;
+  int M = 0;
+
+  size_t footprint() const {
+    return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 + der.mcap * (size_t)der.mpoints * 12 +
+           (size_t)rng.cap * sizeof(int2) + run.partial_cap * sizeof(float2);
+  }
+  void release_buffers() {
+    (void)hipFree(freq); (void)hipFree(out); (void)hipFree(ib); (void)hipFree(ie);
+    freq = out = nullptr;
+    ib = ie = nullptr;
+    freq_cap = out_cap = win_cap = 0;
+    der.release();
+    rng.release();
+    run.release();
+  }
+};
+
+bool compatible(const gx_request& a, const gx_request& b) {
+  return a.profile == b.profile && a.precision == b.precision && a.start_nu == b.start_nu && a.end_nu == b.end_nu &&
+         a.npoints == b.npoints && reference_cutoff(a.profile, a.pressure, a.cutoff) ==
+         reference_cutoff(b.profile, b.pressure, b.cutoff) && a.doppler_widths == b.doppler_widths;
+}
+
+bool mixed_req(const gx_request& r) { return r.profile == GX_VOIGT && r.precision == GX_PREC_MIXED; }
+
+template <class T>
+int grow(T*& p, size_t& cap, size_t n, const char* what) {
+  if (n <= cap) return GX_OK;
+  (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1));
+  if (e) return hip_fail(e, what);
+  cap = std::max<size_t>(n, 1);
+  return GX_OK;
+}
+
+}  // namespace
+
+struct gx_server {
+  gx_lines* L = nullptr;
+  gx_server_opts o{};
+  std::vector<std::unique_ptr<Slot>> slots;
+  size_t budget = 0;
+  int cus = 1;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> pending;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+
+  // statistics (under mu)
+  long long submitted = 0, completed = 0, cancelled = 0, failed = 0, batches = 0, batched = 0, evals = 0;
+  std::vector<double> latencies_ms;
+  bool started = false;
+  Clock::time_point t_first, t_last;
+  size_t peak_bytes = 0;
+};
+
+namespace {
+
+size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& batch, size_t nlines) {
+  const gx_request& r = batch[0]->r;
+  const size_t M = (size_t)r.npoints, B = batch.size();
+  size_t need = std::max(sl.freq_cap, M) * 8 + std::max(sl.out_cap, M * B) * 8 + std::max(sl.win_cap, nlines) * 8;
+  if (mixed_req(r)) {
+    need += std::max(sl.der.mcap * (size_t)sl.der.mpoints, nlines * B) * 12;
+    const size_t nb = (M + 1023) / 1024 + 1;
+    need += nb * B * 16 * 1024 * sizeof(float2) / 4;   // partials, upper estimate
+  } else {
+    need += std::max(sl.der.cap, nlines) * 32;
+  }
+  (void)S;
+  return need;
+}
+
+// Line slice [lo, hi) of the sorted list that can reach grid [start, start + M*d) with cutoff c.
+void line_slice(const gx_lines* L, double start, double end, double cmax, double d, size_t& lo, size_t& hi) {
+  const double a = start - cmax - 2 * d, b = end + cmax + 2 * d;
+  lo = std::lower_bound(L->h_nu.begin(), L->h_nu.end(), a) - L->h_nu.begin();
+  hi = std::upper_bound(L->h_nu.begin(), L->h_nu.end(), b) - L->h_nu.begin();
+}
+
+int launch_batch(gx_server* S, Slot& sl) {
+  const gx_request& r0 = sl.reqs[0]->r;
+  const int M = r0.npoints;
+  const int B = (int)sl.reqs.size();
+  const double d = (r0.end_nu - r0.start_nu) / double(M);     // BaseProfile.cpp:10
+  hipStream_t s = sl.stream;
+  hipError_t e;
+  int rc;
+  // per-request constants
+  sl.h_pcs.resize(B);
+  for (int k = 0; k < B; ++k) {
+    const gx_request& q = sl.reqs[k]->r;
+    gx_consts c{};
+    c.temperature = q.temperature;
+    c.partition = q.partition;
+    c.dfreq = d;
+    c.mean_mass = S->o.mean_mass;
+    c.pressure = q.pressure;
+    c.ref_temp = S->o.ref_temp;
+    c.cutoff = q.cutoff;
+    c.doppler_widths = q.doppler_widths;
+    c.convention = S->o.convention;
+    c.precision = q.precision;
+    sl.h_pcs[k] = make_point_consts(q.profile, c, q.pressure);
+  }
+  const PointConsts& pc0 = sl.h_pcs[0];
+  const double cmax = max_cutoff(pc0, S->L->ls.max_nu, d);
+  size_t lo, hi;
+  line_slice(S->L, r0.start_nu, r0.start_nu + M * d, cmax, d, lo, hi);
+  const int n = (int)(hi - lo);
+  const LineStore& ls = S->L->ls;
+  LinesView lv{ls.nu + lo, ls.elow + lo, ls.aif + lo, ls.gns + lo, ls.gam ? ls.gam + lo : nullptr,
+               ls.nexp ? ls.nexp + lo : nullptr, n};
+  if ((rc = grow(sl.freq, sl.freq_cap, (size_t)M, "server: grid"))) return rc;
+  if ((rc = grow(sl.out, sl.out_cap, (size_t)M * B, "server: grids"))) return rc;
+  if (n > 0 && (size_t)n > sl.win_cap) {
+    (void)hipFree(sl.ib);
+    (void)hipFree(sl.ie);
+    sl.ib = sl.ie = nullptr;
+    sl.win_cap = 0;
+    if ((e = hipMalloc(&sl.ib, 4 * (size_t)n)) || (e = hipMalloc(&sl.ie, 4 * (size_t)n)))
+      return hip_fail(e, "server: windows");
+    sl.win_cap = n;
+  }
+  if (!sl.evals && (e = hipMalloc(&sl.evals, sizeof(unsigned long long) * kEvalSlots))) return hip_fail(e, "server");
+  const GridView gv{sl.freq, M, 0, M, r0.start_nu, d};
+  const bool mixed = mixed_req(r0);
+  if (mixed) {
+    if ((rc = sl.der.reserve_mixed((size_t)std::max(n, 1), B))) return rc;
+    if ((rc = sl.der.set_points(sl.h_pcs.data(), B, s))) return rc;
+  } else {
+    if ((rc = sl.der.reserve((size_t)std::max(n, 1)))) return rc;
+  }
+  // launch shape from the request's own grid only (alone == in a mix)
+  const int bpb = mixed ? sl.run.bins_per_block() : kBinsF64;
+  const int nblocks = (M + bpb - 1) / bpb;
+  if ((rc = sl.rng.reserve(nblocks))) return rc;
+  if (mixed) {
+    if ((rc = sl.run.init())) return rc;
+    const int slots = sl.run.cus * mixed_occupancy(sl.run.R);
+    sl.run.forced_nseg = std::min(16, std::max(1, (4 * slots + nblocks - 1) / nblocks));
+    sl.run.max_grid = std::max(1, (slots + S->o.max_inflight - 1) / S->o.max_inflight);
+    std::vector<MixedTask> tasks(B);
+    for (int k = 0; k < B; ++k)
+      tasks[k] = MixedTask{sl.der.m.amp + (size_t)k * sl.der.m.stride, sl.der.m.y + (size_t)k * sl.der.m.stride,
+                           sl.h_pcs[k].dpwcoef, sl.out + (size_t)k * M};
+    if ((rc = sl.run.prepare(tasks.data(), B, nblocks, s))) return rc;
+  }
+  if ((e = hipEventRecord(sl.ev_start, s))) return hip_fail(e, "server: event");
+  if ((e = launch_grid(sl.freq, M, r0.start_nu, d, s))) return hip_fail(e, "server: grid");
+  if ((e = launch_fill(sl.out, (size_t)M * B, 0.0, s))) return hip_fail(e, "server: fill");
+  if ((e = launch_fill(reinterpret_cast<double*>(sl.evals), kEvalSlots, 0.0, s))) return hip_fail(e, "server: fill");
+  if (n > 0) {
+    const WindowArgs wa{pc0.cutoff, pc0.convention, 1};
+    if (mixed) {
+      if ((e = launch_windows(wa, gv, lv.nu, n, sl.ib, sl.ie, sl.evals, s))) return hip_fail(e, "server: windows");
+      if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, sl.der.m, s))) return hip_fail(e, "server: prep");
+      if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
+      if ((rc = sl.run.launch(gv, lv.nu, sl.rng.r, sl.ib, sl.ie, s))) return rc;
+    } else {
+      if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
+      for (int k = 0; k < B; ++k) {
+        if ((e = launch_line_prep_f64(sl.h_pcs[k], gv, lv, sl.der.d, k == 0 ? sl.evals : nullptr, s)))
+          return hip_fail(e, "server: prep");
+        if ((e = launch_accum_f64(sl.h_pcs[k], gv, lv.nu, sl.der.d, sl.rng.r, nblocks, sl.out + (size_t)k * M, s)))
+          return hip_fail(e, "server: accumulate");
+      }
+    }
+  }
+  if ((e = hipEventRecord(sl.ev_done, s))) return hip_fail(e, "server: event");
+  sl.M = M;
+  return GX_OK;
+}
+
+void retire(gx_server* S, Slot& sl) {
+  const int B = (int)sl.reqs.size();
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, sl.ev_start, sl.ev_done);
+  unsigned long long slots[kEvalSlots] = {0};
+  hipError_t e = hipMemcpy(slots, sl.evals, sizeof(slots), hipMemcpyDeviceToHost);
+  unsigned long long ev = 0;
+  for (auto v : slots) ev += v;
+  std::vector<std::vector<double>> results(B);
+  bool ok = e == hipSuccess;
+  for (int k = 0; k < B && ok; ++k) {
+    if (sl.reqs[k]->cancel) continue;
+    results[k].resize(sl.M);
+    if (hipMemcpy(results[k].data(), sl.out + (size_t)k * sl.M, sizeof(double) * sl.M, hipMemcpyDeviceToHost))
+      ok = false;
+  }
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (int k = 0; k < B; ++k) {
+    Req* q = sl.reqs[k];
+    q->t_done = now;
+    q->kernel_ms = ms;
+    if (q->cancel) {
+      q->state = GX_REQ_CANCELLED;
+      ++S->cancelled;
+    } else if (!ok) {
+      q->state = GX_REQ_FAILED;
+      q->error = "device copy failed";
+      ++S->failed;
+    } else {
+      q->host.swap(results[k]);
+      q->evals = (long long)ev;
+      q->state = GX_REQ_DONE;
+      ++S->completed;
+      S->evals += (long long)ev;
+      S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - q->t_submit).count());
+    }
+  }
+  S->t_last = now;
+  sl.reqs.clear();
+  sl.busy = false;
+  S->cv_done.notify_all();
+}
+
+void scheduler(gx_server* S) {
+  (void)hipSetDevice(S->L->device);
+  for (;;) {
+    bool progressed = false;
+    // 1. retire finished batches
+    for (auto& up : S->slots) {
+      Slot& sl = *up;
+      if (sl.busy && hipEventQuery(sl.ev_done) == hipSuccess) {
+        retire(S, sl);
+        progressed = true;
+      }
+    }
+    // 2. admit
+    for (auto& up : S->slots) {
+      Slot& sl = *up;
+      if (sl.busy) continue;
+      std::vector<Req*> batch;
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        while (!S->pending.empty()) {
+          auto it = S->reqs.find(S->pending.front());
+          if (it == S->reqs.end() || it->second->cancel) {
+            if (it != S->reqs.end()) {
+              it->second->state = GX_REQ_CANCELLED;
+              ++S->cancelled;
+              S->cv_done.notify_all();
+            }
+            S->pending.pop_front();
+            continue;
+          }
+          break;
+        }
+        if (S->pending.empty()) break;
+        Req* head = S->reqs[S->pending.front()].get();
+        batch.push_back(head);
+        for (size_t i = 1; i < S->pending.size() && (int)batch.size() < S->o.max_batch; ++i) {
+          auto it = S->reqs.find(S->pending[i]);
+          if (it != S->reqs.end() && !it->second->cancel && compatible(it->second->r, head->r))
+            batch.push_back(it->second.get());
+        }
+        // HBM admission: footprint of the other slots plus this slot after growth
+        const gx_request& r = head->r;
+        const double d = (r.end_nu - r.start_nu) / double(r.npoints);
+        gx_consts c{};
+        c.temperature = r.temperature; c.partition = r.partition; c.dfreq = d; c.mean_mass = S->o.mean_mass;
+        c.pressure = r.pressure; c.ref_temp = S->o.ref_temp; c.cutoff = r.cutoff; c.doppler_widths = r.doppler_widths;
+        const PointConsts pc = make_point_consts(r.profile, c, r.pressure);
+        size_t lo, hi;
+        line_slice(S->L, r.start_nu, r.end_nu, max_cutoff(pc, S->L->ls.max_nu, d), d, lo, hi);
+        size_t others = 0;
+        for (auto& o2 : S->slots) if (o2.get() != &sl) others += o2->footprint();
+        size_t need = batch_need(S, sl, batch, hi - lo);
+        while (batch.size() > 1 && others + need > S->budget) {   // shrink the batch first
+          batch.pop_back();
+          need = batch_need(S, sl, batch, hi - lo);
+        }
+        if (others + need > S->budget) {
+          bool any_busy = false;
+          for (auto& o2 : S->slots) any_busy |= o2->busy;
+          if (any_busy) break;                                     // wait for memory to free up
+          if (need > S->budget) {
+            head->state = GX_REQ_FAILED;
+            head->error = "request does not fit the HBM budget";
+            ++S->failed;
+            S->pending.pop_front();
+            S->cv_done.notify_all();
+            progressed = true;
+            continue;
+          }
+        }
+        for (Req* q : batch) {
+          q->state = GX_REQ_RUNNING;
+          S->pending.erase(std::find(S->pending.begin(), S->pending.end(), q->id));
+        }
+        if (!S->started) {
+          S->started = true;
+          S->t_first = Clock::now();
+        }
+        ++S->batches;
+        S->batched += (long long)batch.size();
+        S->peak_bytes = std::max(S->peak_bytes, others + need);
+      }
+      sl.reqs = batch;
+      sl.busy = true;
+      sl.t_launch = Clock::now();
+      const int rc = launch_batch(S, sl);
+      if (rc) {
+        std::string msg = gx_last_error();
+        (void)hipStreamSynchronize(sl.stream);
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (Req* q : sl.reqs) {
+          q->state = GX_REQ_FAILED;
+          q->error = msg;
+          ++S->failed;
+        }
+        sl.reqs.clear();
+        sl.busy = false;
+        S->cv_done.notify_all();
+      }
+      progressed = true;
+    }
+    std::unique_lock<std::mutex> lk(S->mu);
+    bool any_busy = false;
+    for (auto& up : S->slots) any_busy |= up->busy;
+    if (S->stop && !any_busy && S->pending.empty()) break;
+    if (!progressed) {
+      if (any_busy)
+        S->cv_work.wait_for(lk, std::chrono::microseconds(100));
+      else
+        S->cv_work.wait_for(lk, std::chrono::milliseconds(20), [&] { return S->stop || !S->pending.empty(); });
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_server_create(gx_lines* L, const gx_server_opts* o, gx_server** out) {
+  if (!out) return fail(GX_INVALID_ARG, "server_create: out is NULL");
+  *out = nullptr;
+  if (!L || !o || !(o->mean_mass > 0) || !(o->ref_temp > 0)) return fail(GX_INVALID_ARG, "server_create: bad arguments");
+  hipError_t e;
+  if ((e = hipSetDevice(L->device))) return hip_fail(e, "hipSetDevice");
+  gx_server* S = new gx_server();
+  S->L = L;
+  S->o = *o;
+  if (S->o.max_inflight <= 0) S->o.max_inflight = 4;
+  if (S->o.max_batch <= 0) S->o.max_batch = 16;
+  size_t free_b = 0, total_b = 0;
+  if ((e = hipMemGetInfo(&free_b, &total_b))) { delete S; return hip_fail(e, "hipMemGetInfo"); }
+  const size_t reserve = size_t(4) << 30;
+  S->budget = o->hbm_budget ? o->hbm_budget : (free_b > reserve ? free_b - reserve : free_b / 2);
+  for (int k = 0; k < S->o.max_inflight; ++k) {
+    auto sl = std::make_unique<Slot>();
+    if ((e = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking)) || (e = hipEventCreate(&sl->ev_start)) ||
+        (e = hipEventCreate(&sl->ev_done))) {
+      delete S;
+      return hip_fail(e, "server_create: streams");
+    }
+    S->slots.push_back(std::move(sl));
+  }
+  S->worker = std::thread(scheduler, S);
+  *out = S;
+  return GX_OK;
+}
+
+int gx_server_submit(gx_server* S, const gx_request* r, uint64_t* id) {
+  if (!S || !r || !id) return fail(GX_INVALID_ARG, "server_submit: NULL argument");
+  if (r->profile != GX_VOIGT && r->profile != GX_DOPPLER) return fail(GX_INVALID_ARG, "server_submit: bad profile");
+  if (r->profile == GX_VOIGT && !S->L->voigt) return fail(GX_INVALID_ARG, "server_submit: lines have no broadening");
+  if (r->npoints <= 0 || !(r->end_nu > r->start_nu) || !(r->temperature > 0) || !(r->partition > 0) ||
+      !std::isfinite(r->pressure))
+    return fail(GX_INVALID_ARG, "server_submit: bad request");
+  auto q = std::make_unique<Req>();
+  q->r = *r;
+  if (q->r.profile == GX_DOPPLER) q->r.precision = GX_PREC_FP64;
+  q->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return fail(GX_BUSY, "server_submit: server is stopping");
+    q->id = S->next_id++;
+    *id = q->id;
+    S->pending.push_back(q->id);
+    S->reqs[q->id] = std::move(q);
+    ++S->submitted;
+  }
+  S->cv_work.notify_one();
+  return GX_OK;
+}
+
+int gx_server_poll(gx_server* S, uint64_t id, int* state) {
+  if (!S || !state) return fail(GX_INVALID_ARG, "server_poll: NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(GX_NOT_FOUND, "server_poll: unknown request");
+  *state = it->second->state;
+  return GX_OK;
+}
+
+int gx_server_wait(gx_server* S, uint64_t id, double timeout_s, int* state) {
+  if (!S) return fail(GX_INVALID_ARG, "server_wait: NULL server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto done = [&] {
+    auto it = S->reqs.find(id);
+    return it == S->reqs.end() || it->second->state >= GX_REQ_DONE;
+  };
+  if (timeout_s < 0)
+    S->cv_done.wait(lk, done);
+  else
+    S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(GX_NOT_FOUND, "server_wait: unknown request");
+  if (state) *state = it->second->state;
+  return GX_OK;
+}
+
+int gx_server_result(gx_server* S, uint64_t id, double* out, int n, gx_timing* t) {
+  if (!S || !out) return fail(GX_INVALID_ARG, "server_result: NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(GX_NOT_FOUND, "server_result: unknown request");
+  Req* q = it->second.get();
+  if (q->state == GX_REQ_FAILED) return fail(GX_DEVICE_ERROR, "server_result: request failed: " + q->error);
+  if (q->state != GX_REQ_DONE) return fail(GX_BUSY, "server_result: request not finished");
+  if (n != q->r.npoints) return fail(GX_INVALID_ARG, "server_result: n differs from the request's npoints");
+  std::copy(q->host.begin(), q->host.end(), out);
+  if (t) {
+    t->intensity_ms = 0;
+    t->accumulate_ms = q->kernel_ms;
+    t->total_ms = (float)std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count();
+    t->evals = q->evals;
+    t->lines = (long long)S->L->ls.n;
+  }
+  S->reqs.erase(it);
+  return GX_OK;
+}
+
+int gx_server_cancel(gx_server* S, uint64_t id) {
+  if (!S) return fail(GX_INVALID_ARG, "server_cancel: NULL server");
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(GX_NOT_FOUND, "server_cancel: unknown request");
+    if (it->second->state >= GX_REQ_DONE) return GX_OK;
+    it->second->cancel = true;
+  }
+  S->cv_work.notify_one();
+  return GX_OK;
+}
+
+int gx_server_stats(gx_server* S, gx_server_summary* out) {
+  if (!S || !out) return fail(GX_INVALID_ARG, "server_stats: NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = gx_server_summary{};
+  out->submitted = S->submitted;
+  out->completed = S->completed;
+  out->cancelled = S->cancelled;
+  out->failed = S->failed;
+  out->batches = S->batches;
+  out->batched_requests = S->batched;
+  out->evals = S->evals;
+  out->peak_bytes = S->peak_bytes;
+  if (S->started) out->busy_s = std::chrono::duration<double>(S->t_last - S->t_first).count();
+  std::vector<double> v = S->latencies_ms;
+  if (!v.empty()) {
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double p) {
+      const double pos = p * (double)(v.size() - 1);
+      const size_t i = (size_t)pos;
+      const double f = pos - (double)i;
+      return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+    };
+    out->p50_ms = pct(0.50);
+    out->p99_ms = pct(0.99);
+    double s = 0;
+    for (double x : v) s += x;
+    out->mean_ms = s / (double)v.size();
+    out->max_ms = v.back();
+  }
+  return GX_OK;
+}
+
+int gx_server_destroy(gx_server* S) {
+  if (!S) return GX_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    for (auto id : S->pending) {
+      auto it = S->reqs.find(id);
+      if (it != S->reqs.end()) it->second->cancel = true;
+    }
+  }
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (auto& sl : S->slots) {
+    (void)hipStreamSynchronize(sl->stream);
+    sl->release_buffers();
+    (void)hipFree(sl->evals);
+    (void)hipEventDestroy(sl->ev_start);
+    (void)hipEventDestroy(sl->ev_done);
+    (void)hipStreamDestroy(sl->stream);
+  }
+  delete S;
+  return GX_OK;
+}
+
+}  // extern "C"

@@ -33,13 +33,6 @@
 
 using namespace gx;
 
-struct gx_lines {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  LineStore ls;
-  bool voigt = false;
-};
-
 namespace {
 
 struct Group {                // points that share one cutoff (hence windows and block ranges)
@@ -272,6 +265,11 @@ int gx_lines_create(int device, size_t n, const double* energies, const double* 
   if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking))) { delete L; return hip_fail(e, "stream"); }
   int rc = L->ls.upload(n, energies, nu, aif, gns, gamma_ref, nexp, L->stream);
   if (rc) { gx_lines_destroy(L); return rc; }
+  L->h_nu.resize(n);
+  if ((e = hipMemcpy(L->h_nu.data(), L->ls.nu, n * sizeof(double), hipMemcpyDeviceToHost))) {
+    gx_lines_destroy(L);
+    return hip_fail(e, "lines_create: copy back");
+  }
   *out = L;
   return GX_OK;
 }

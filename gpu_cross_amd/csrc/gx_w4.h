@@ -88,21 +88,24 @@ __device__ __forceinline__ double w4(double x, double y) {
 // FP32 forms used by the mixed-precision kernels.
 // ---------------------------------------------------------------------------
 
-// Regions II-III in FP32 and region IV in FP64 (its exp(u) - tP/Q difference
-// cancels by up to two digits near the Doppler core).
-__device__ __noinline__ float w4_inner_f32(float x, float y) {
-  const float s = x + y;
-  if (s >= 5.5f) {  // region II
+// W4 regions II-IV for the mixed-precision core bins (|x| + y < 15), x >= 0.
+// Regions II and III are rational in t with no cancellation and run in FP32
+// (max 1.6e-6 relative against FP64 over their domains); region IV keeps the
+// FP64 t P(u)/Q(u) term, whose real part cancels in FP32 near |x| ~ 5, and
+// takes exp(u) = e^(y^2-x^2)(cos 2xy - i sin 2xy) from the FP32 hardware
+// exp/cos (2.7e-6 max).  The region is decided in FP64 by the caller's x, y.
+__device__ __noinline__ float w4_core_mixed(double xd, double yd) {
+  const double sd = xd + yd;
+  const float x = (float)xd, y = (float)yd;
+  if (sd >= 5.5) {  // region II: t (1.410474 + 0.5641896 u) / (0.75 + u (3 + u))
     const float ur = y * y - x * x, ui = -2.0f * x * y;
-    // num = t (1.410474 + 0.5641896 u)
     const float ar = 1.410474f + 0.5641896f * ur, ai = 0.5641896f * ui;
-    const float nr = y * ar + x * ai, ni = y * ai - x * ar;
-    // den = 0.75 + u (3 + u)
+    const float nr = y * ar + x * ai, ni = y * ai - x * ar;            // t * a, t = y - i x
     const float br = 3.0f + ur, bi = ui;
     const float dr = 0.75f + (ur * br - ui * bi), di = ur * bi + ui * br;
-    return (nr * dr + ni * di) / (dr * dr + di * di);
+    return (nr * dr + ni * di) * __builtin_amdgcn_rcpf(dr * dr + di * di);
   }
-  if (y >= 0.195f * x - 0.176f) {  // region III
+  if (yd >= 0.195 * xd - 0.176) {  // region III
     const float tr = y, ti = -x;
     float nr = tr * 0.5642236f, ni = ti * 0.5642236f;
     float ar, ai;
@@ -116,9 +119,32 @@ __device__ __noinline__ float w4_inner_f32(float x, float y) {
     er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 39.27121f + er; di = ei;
     er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 38.82363f + er; di = ei;
     er = tr * dr - ti * di; ei = tr * di + ti * dr; dr = 16.4955f + er; di = ei;
-    return (nr * dr + ni * di) / (dr * dr + di * di);
+    return (nr * dr + ni * di) * __builtin_amdgcn_rcpf(dr * dr + di * di);
   }
-  return (float)w4_inner((double)x, (double)y);
+  // region IV: exp(u) - t P(u) / Q(u)
+  const cplx t{yd, -xd};
+  const cplx u = cmul(t, t);
+  cplx p = cscale(u, 0.56419);
+  p = csub(1.320522, p);
+  p = csub(35.76683, cmul(u, p));
+  p = csub(219.0313, cmul(u, p));
+  p = csub(1540.787, cmul(u, p));
+  p = csub(3321.9905, cmul(u, p));
+  p = csub(36183.31, cmul(u, p));
+  cplx q = csub(1.841439, u);
+  q = csub(61.57037, cmul(u, q));
+  q = csub(364.2191, cmul(u, q));
+  q = csub(2186.181, cmul(u, q));
+  q = csub(9022.228, cmul(u, q));
+  q = csub(24322.84, cmul(u, q));
+  q = csub(32066.6, cmul(u, q));
+  const cplx n = cmul(t, p);
+  const double qq = q.re * q.re + q.im * q.im;
+  double rq = __builtin_amdgcn_rcp(qq);
+  rq = rq * __builtin_fma(-qq, rq, 2.0);          // one Newton step on the hardware reciprocal
+  const double tpq = (n.re * q.re + n.im * q.im) * rq;
+  const float ex = __expf((float)u.re) * __cosf((float)u.im);
+  return (float)((double)ex - tpq);
 }
 
 }  // namespace gx

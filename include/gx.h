@@ -178,6 +178,64 @@ int gx_table_read(gx_table *table, int point, double *out, int n);
 int gx_table_shape(gx_table *table, int *bins_per_lane, int *segments);
 int gx_table_destroy(gx_table *table);
 
+/* ------------------------------------------------------------------------
+ * Request server: continuous batching of cross-section requests over one
+ * resident line list.  No reference counterpart (the reference is one
+ * cross-section per process).  A scheduler thread admits queued requests as
+ * in-flight slots and HBM budget free up, co-batches compatible ones (same
+ * grid, profile, precision, cutoff) into one launch, runs batches
+ * concurrently on separate streams, and records per-request latency.  A
+ * request's result does not depend on what it was batched with.
+ * ------------------------------------------------------------------------ */
+typedef struct gx_server gx_server;
+
+typedef struct gx_request {
+    double temperature;     /* K */
+    double pressure;        /* atm (multiplies gamma_ref) */
+    double partition;       /* Q(T) */
+    int profile;            /* GX_VOIGT or GX_DOPPLER */
+    int precision;          /* GX_PREC_* (Doppler runs FP64) */
+    double start_nu;        /* grid, BaseProfile.cpp:7-21 rule */
+    double end_nu;
+    int npoints;
+    double cutoff;          /* <= 0: reference rule */
+    double doppler_widths;  /* Doppler: > 0 per-line window k*alpha + dfreq */
+} gx_request;
+
+typedef struct gx_server_opts {
+    double mean_mass;       /* amu */
+    double ref_temp;        /* K */
+    int convention;         /* GX_CONV_* */
+    int max_inflight;       /* batches on the device at once (0: 4) */
+    int max_batch;          /* requests per batch (0: 16) */
+    size_t hbm_budget;      /* bytes for in-flight work (0: free HBM less 4 GiB) */
+} gx_server_opts;
+
+enum { GX_REQ_QUEUED = 0, GX_REQ_RUNNING = 1, GX_REQ_DONE = 2, GX_REQ_FAILED = 3, GX_REQ_CANCELLED = 4 };
+
+typedef struct gx_server_summary {
+    long long submitted, completed, cancelled, failed, batches, batched_requests;
+    long long evals;        /* line-bin evaluations of completed requests */
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> result ready, completed requests */
+    double busy_s;          /* wall time from first admission to last completion */
+    size_t peak_bytes;      /* largest in-flight device footprint */
+} gx_server_summary;
+
+int gx_server_create(gx_lines *lines, const gx_server_opts *opts, gx_server **out);
+/* Queue a request; id identifies it in the calls below. */
+int gx_server_submit(gx_server *srv, const gx_request *req, uint64_t *id);
+/* Current state (GX_REQ_*). */
+int gx_server_poll(gx_server *srv, uint64_t id, int *state);
+/* Block until the request leaves the queue/device or timeout_s passes (< 0: no limit). */
+int gx_server_wait(gx_server *srv, uint64_t id, double timeout_s, int *state);
+/* Copy a finished grid (n bins) and release the request; timing may be NULL. */
+int gx_server_result(gx_server *srv, uint64_t id, double *out, int n, gx_timing *timing);
+/* Cancel a queued or running request (its result is discarded). */
+int gx_server_cancel(gx_server *srv, uint64_t id);
+int gx_server_stats(gx_server *srv, gx_server_summary *out);
+/* Stop the scheduler after the in-flight batches finish and free everything. */
+int gx_server_destroy(gx_server *srv);
+
 /* Message of the last failing call on this host thread. */
 const char *gx_last_error(void);
 

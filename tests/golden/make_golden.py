@@ -12,7 +12,6 @@ from __future__ import annotations
 
 import json
 import os
-import struct
 import subprocess
 import sys
 import tempfile
@@ -24,30 +23,19 @@ sys.path.insert(0, ROOT)
 
 from gpu_cross_amd.synth import make_lines  # noqa: E402
 from oracle import pyoracle  # noqa: E402
+from tests.golden.caseio import read_grid, write_case  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 
 
 def run_reference(case: dict, lines: dict) -> np.ndarray:
-    prof = case["profile"]
-    n = lines["nu"].size
-    case["capacity"] = min(case["capacity"], n)      # keeps the reference's host buffers small
+    case["capacity"] = min(case["capacity"], lines["nu"].size)   # keeps the reference's host buffers small
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        with open(fin, "wb") as f:
-            f.write(struct.pack("<5i", prof, case["nthreads"], case["npoints"], case["max_points"], case["capacity"]))
-            f.write(struct.pack("<7d", case["start"], case["end"], case["T"], case["Q"], case["mass"], case["P"],
-                                case["ref_temp"]))
-            f.write(struct.pack("<q", n))
-            for k in ("elow", "nu", "aif"):
-                f.write(lines[k].astype("<f8").tobytes())
-            f.write(lines["gns"].astype("<i4").tobytes())
-            if prof == pyoracle.VOIGT:
-                f.write(lines["gamma"].astype("<f8").tobytes())
-                f.write(lines["nexp"].astype("<f8").tobytes())
+        write_case(fin, case, lines)
         subprocess.run([HARNESS, fin, fout], check=True, stdout=subprocess.DEVNULL)
-        return np.fromfile(fout, dtype="<f8")
+        return read_grid(fout)
 
 
 def select(ll, lo, hi, order="sorted"):

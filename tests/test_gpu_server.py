@@ -1,0 +1,100 @@
+"""The request server on the GPU: every result equals the same request run
+alone (bit for bit), matches the oracle, and cancellation / admission
+failures are reported per request."""
+import numpy as np
+import pytest
+
+from gpu_cross_amd import capi
+from gpu_cross_amd.manager import make_grid
+from gpu_cross_amd.server import Server
+from gpu_cross_amd.synth import make_lines
+from gpu_cross_amd.table import ResidentLines
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def world():
+    ll = make_lines(31, 20000, 20000.0, 400_000, 12000.0, 17.03)
+    return ll, ResidentLines.from_linelist(ll)
+
+
+def requests(ll):
+    Q = lambda T: po.partition(ll.state_e, ll.state_g, T)
+    grids = [(1000.0, 1040.0, 8000), (2500.0, 2520.0, 2000), (6000.0, 6100.0, 20000)]
+    reqs = []
+    rng = np.random.default_rng(3)
+    for k in range(14):
+        g = grids[k % 3]
+        T = float(rng.uniform(200, 3000))
+        P = float(10 ** rng.uniform(-3, 1))
+        if k % 5 == 4:
+            reqs.append(dict(T=T, P=P, Q=Q(T), grid=g, profile=capi.DOPPLER, precision=capi.PREC_FP64, cutoff=0.0,
+                             doppler_widths=25.0))
+        elif k % 5 == 3:
+            reqs.append(dict(T=T, P=P, Q=Q(T), grid=g, profile=capi.VOIGT, precision=capi.PREC_FP64, cutoff=25.0,
+                             doppler_widths=0.0))
+        else:
+            reqs.append(dict(T=T, P=P, Q=Q(T), grid=g, profile=capi.VOIGT, precision=capi.PREC_MIXED, cutoff=25.0,
+                             doppler_widths=0.0))
+    return reqs
+
+
+def submit(srv, r):
+    return srv.submit(r["T"], r["P"], r["Q"], *r["grid"], profile=r["profile"], precision=r["precision"],
+                      cutoff=r["cutoff"], doppler_widths=r["doppler_widths"])
+
+
+def test_alone_equals_mix_and_oracle(world):
+    ll, lines = world
+    reqs = requests(ll)
+    print(f"server: {len(reqs)} requests over {len(lines)} lines", flush=True)
+    with Server(lines, 17.03, max_inflight=3, max_batch=8) as srv:
+        ids = [submit(srv, r) for r in reqs]                    # all arrive together: co-batched, overlapping
+        mix = []
+        for rid in ids:
+            assert srv.wait(rid, 120.0) == capi.REQ_DONE
+            mix.append(srv.result(rid)[0])
+        st = srv.stats()
+        assert st.completed == len(reqs) and st.failed == 0
+        assert st.batches < len(reqs)                           # some requests shared a launch
+        assert st.p99_ms >= st.p50_ms > 0
+    with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
+        for r, m in zip(reqs, mix):
+            rid = submit(srv, r)
+            assert srv.wait(rid, 120.0) == capi.REQ_DONE
+            alone = srv.result(rid)[0]
+            assert np.array_equal(alone, m)
+    # spot checks against the oracle
+    for r, m in list(zip(reqs, mix))[:5]:
+        start, end, npts = r["grid"]
+        freq, d = make_grid(start, end, npts)
+        voigt = r["profile"] == capi.VOIGT
+        c = po.consts(r["profile"], r["T"], r["Q"], d, 17.03, r["P"], 500.0 if voigt else 500.0,
+                      cutoff=r["cutoff"] if voigt else 100.0, doppler_widths=r["doppler_widths"])
+        sel = np.nonzero((ll.nu > start - 120) & (ll.nu < end + 120))[0]
+        L = ll.subset(sel)
+        ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, L.gamma_at(r["P"]) if voigt else None,
+                               L.nexp if voigt else None, freq, npts, 0, 8)
+        mask = ref >= 1e-6 * ref.max()
+        err = float(np.max(np.abs(m[mask] - ref[mask]) / ref[mask]))
+        tol = 1e-5 if r["precision"] == capi.PREC_MIXED else 1e-10
+        print(f"  request profile={r['profile']} prec={r['precision']}: max rel err {err:.2e}", flush=True)
+        assert err <= tol
+
+
+def test_cancel_and_budget_rejection(world):
+    ll, lines = world
+    Q = po.partition(ll.state_e, ll.state_g, 1000.0)
+    with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
+        ids = [srv.submit(1000.0, 1.0, Q, 100.0, 11000.0, 200_000) for _ in range(4)]
+        srv.cancel(ids[-1])
+        states = [srv.wait(rid, 300.0) for rid in ids]
+        assert states[:3] == [capi.REQ_DONE] * 3
+        assert states[3] in (capi.REQ_CANCELLED, capi.REQ_DONE)
+    with Server(lines, 17.03, hbm_budget=1 << 20) as srv:       # 1 MiB: nothing fits
+        rid = srv.submit(1000.0, 1.0, Q, 100.0, 11000.0, 200_000)
+        assert srv.wait(rid, 60.0) == capi.REQ_FAILED
+        with pytest.raises(capi.GxError):
+            srv.result(rid)
