@@ -316,8 +316,8 @@ struct MixedShape {
 
 struct alignas(16) StagedLine {
   int ib, ie, k0, k1;         // window and core ranges (absolute bins)
-  float r, e, a1, a2;         // fast-path operands
-  float rr, p2, pad, cf;      // cf: centre bin relative to the block origin
+  float r, e, h, k;           // offset operands; h = y^2 - 1/2, k = 2 y^2 (scaled)
+  float a, b, pad, cf;        // a = amp c y, b = a (scaled); cf: centre bin relative to the block origin
 };
 
 __device__ __forceinline__ void kahan_step(float& sum, float& comp, float& part) {
@@ -328,10 +328,14 @@ __device__ __forceinline__ void kahan_step(float& sum, float& comp, float& part)
   part = 0.0f;
 }
 
-__device__ __forceinline__ float region1(float x, float a1, float a2, float rr, float p2) {
-  const float x2 = x * x;
-  const float num = __builtin_fmaf(a2, x2, a1);
-  const float den = __builtin_fmaf(x2, x2 + rr, p2);
+// W4 region I, c y (1/2 + x^2 + y^2) / ((1/2 + y^2 - x^2)^2 + 4 x^2 y^2), rewritten with
+// u = x^2 + y^2 - 1/2 as c y (u + 1) / (u^2 + 2 y^2): the same rational function (the
+// quartic's constant term reduces to 2y^2), 4 FMA + 1 reciprocal.  In region I
+// |x| >= 15 - y, so u has no cancellation.
+__device__ __forceinline__ float region1(float x, float h, float k, float a, float b) {
+  const float u = __builtin_fmaf(x, x, h);
+  const float den = __builtin_fmaf(u, u, k);
+  const float num = __builtin_fmaf(a, u, b);
   return num * __builtin_amdgcn_rcpf(den);
 }
 
@@ -405,11 +409,13 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
         L.r = (float)((nu - fc) * g * sg);
         L.e = e;
         L.pad = 0.0f;
+        // scaled: u' = x'^2 + (y^2 - 1/2) s^2, den' = u'^2 + 2 y^2 s^4, num' = a s^2 (u' + s^2)
         const double y2 = y * y;
-        L.a2 = (float)((double)amp * 0.5641896 * y * sg2);
-        L.a1 = (float)((double)amp * 0.5641896 * y * (0.5 + y2) * sg4);
-        L.rr = (float)((2.0 * y2 - 1.0) * sg2);
-        L.p2 = (float)((0.5 + y2) * (0.5 + y2) * sg4);
+        const double ac = (double)amp * 0.5641896 * y;
+        L.h = (float)((y2 - 0.5) * sg2);
+        L.k = (float)(2.0 * y2 * sg4);
+        L.a = (float)(ac * sg2);
+        L.b = (float)(ac * sg4);
         L.cf = (float)(cj - (double)j0);
         // core bins: |nu - f_j| g + y < 15, as an exact integer range [k0, k1)
         int k0 = 0, k1 = 0;
@@ -458,21 +464,18 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
         fast_m[m] = __ballot(f);
         edge_m[m] = __ballot(ed);
       }
-      // fast lines: branch-free FP32 region I for all R bins of every lane.  The next
-      // line's record is read while this one is evaluated.
+      // fast lines: branch-free FP32 region I for all R bins of every lane.  Two records
+      // are in flight (ping-pong): the LDS read of one line overlaps the other's math.
 #pragma unroll
       for (int m = 0; m < kTile / 64; ++m) {
         unsigned long long bits = fast_m[m];
         if (!bits) continue;
-        int k = m * 64 + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        float4 F = *reinterpret_cast<const float4*>(&s_l[k].r);
-        float4 G = *reinterpret_cast<const float4*>(&s_l[k].rr);
-        for (;;) {
-          const bool more = bits != 0;
-          const int kn = more ? m * 64 + __builtin_ctzll(bits) : k;
-          const float4 Fn = *reinterpret_cast<const float4*>(&s_l[kn].r);
-          const float4 Gn = *reinterpret_cast<const float4*>(&s_l[kn].rr);
+        auto next = [&]() {
+          const int k = m * 64 + __builtin_ctzll(bits);
+          bits &= bits - 1;
+          return k;
+        };
+        auto eval = [&](const float4& F, const float4& G) {
 #pragma unroll
           for (int q = 0; q < R; ++q)
             part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
@@ -481,10 +484,34 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
 #pragma unroll
             for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
           }
-          if (!more) break;
-          bits &= bits - 1;
-          F = Fn;
-          G = Gn;
+        };
+        int ka = next();
+        float4 FA = *reinterpret_cast<const float4*>(&s_l[ka].r);
+        float4 GA = *reinterpret_cast<const float4*>(&s_l[ka].a);
+        float4 FB, GB;
+        bool hb = bits != 0;
+        if (hb) {
+          const int kb = next();
+          FB = *reinterpret_cast<const float4*>(&s_l[kb].r);
+          GB = *reinterpret_cast<const float4*>(&s_l[kb].a);
+        }
+        for (;;) {
+          eval(FA, GA);
+          if (!hb) break;
+          const bool ha = bits != 0;
+          if (ha) {
+            ka = next();
+            FA = *reinterpret_cast<const float4*>(&s_l[ka].r);
+            GA = *reinterpret_cast<const float4*>(&s_l[ka].a);
+          }
+          eval(FB, GB);
+          if (!ha) break;
+          hb = bits != 0;
+          if (hb) {
+            const int kb = next();
+            FB = *reinterpret_cast<const float4*>(&s_l[kb].r);
+            GB = *reinterpret_cast<const float4*>(&s_l[kb].a);
+          }
         }
       }
       // edge lines: per-lane window and core tests; core bins in FP64
@@ -496,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
           bits &= bits - 1;
           const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
           const float4 F = *reinterpret_cast<const float4*>(&s_l[k].r);
-          const float4 G = *reinterpret_cast<const float4*>(&s_l[k].rr);
+          const float4 G = *reinterpret_cast<const float4*>(&s_l[k].a);
 #pragma unroll
           for (int q = 0; q < R; ++q) {
             const int jj = w0 + lane + 64 * q;
