@@ -174,12 +174,162 @@ def bench_table(args, wl):
     print(json.dumps(res), flush=True)
 
 
+def serving_requests(rng, n, ll, Qfun):
+    """Requests of the serving workload: T ~ U[200,3000] K, P log-U[1e-5,100] atm, Doppler or
+    Voigt 50/50, window width log-U[10,12000] cm^-1 placed uniformly in [0,12000], resolution
+    log-U[0.001,0.1] cm^-1, at most 1e7 bins."""
+    from gpu_cross_amd import capi
+    out = []
+    for _ in range(n):
+        T = float(rng.uniform(200.0, 3000.0))
+        P = float(10 ** rng.uniform(-5.0, 2.0))
+        width = float(10 ** rng.uniform(1.0, np.log10(12000.0)))
+        start = float(rng.uniform(0.0, 12000.0 - width))
+        res = float(10 ** rng.uniform(-3.0, -1.0))
+        npts = int(min(max(round(width / res), 1), 10_000_000))
+        voigt = rng.uniform() < 0.5
+        out.append(dict(T=T, P=P, Q=Qfun(T), start=start, end=start + width, npoints=npts,
+                        profile=capi.VOIGT if voigt else capi.DOPPLER,
+                        precision=capi.PREC_MIXED if voigt else capi.PREC_FP64,
+                        cutoff=25.0 if voigt else 0.0, doppler_widths=0.0 if voigt else 25.0))
+    return out
+
+
+def bench_serving(args, wl):
+    """Continuous batching over a resident 3e8-line list with seeded Poisson arrivals."""
+    from gpu_cross_amd import capi
+    from gpu_cross_amd.server import Server
+    from gpu_cross_amd.synth import workload
+    from gpu_cross_amd.table import ResidentLines
+    from oracle import pyoracle as po
+
+    t0 = time.time()
+    ll = workload("serving_3e8")
+    log(f"generated {ll.n} lines in {time.time() - t0:.1f} s")
+    t0 = time.time()
+    lines = ResidentLines.from_linelist(ll)           # sorted on the device
+    log(f"uploaded + sorted on device in {time.time() - t0:.1f} s")
+    qcache = {}
+    Qfun = lambda T: qcache.setdefault(T, po.partition(ll.state_e, ll.state_g, T))
+    submit = lambda srv, r: srv.submit(r["T"], r["P"], r["Q"], r["start"], r["end"], r["npoints"],
+                                       profile=r["profile"], precision=r["precision"], cutoff=r["cutoff"],
+                                       doppler_widths=r["doppler_widths"])
+    rng = np.random.default_rng(55)
+    # capacity: a closed-loop burst of requests
+    calib = serving_requests(rng, 24, ll, Qfun)
+    with Server(lines, ll.mass, max_inflight=4, max_batch=16) as srv:
+        t1 = time.perf_counter()
+        ids = [submit(srv, r) for r in calib]
+        for rid in ids:
+            srv.wait(rid)
+            srv.result(rid)
+        cap_rps = len(calib) / (time.perf_counter() - t1)
+    log(f"closed-loop capacity ~{cap_rps:.2f} requests/s")
+    levels = []
+    keep = []
+    for load in (0.5, 0.9):
+        reqs = serving_requests(rng, args.requests, ll, Qfun)
+        gaps = rng.exponential(1.0 / (load * cap_rps), len(reqs))
+        with Server(lines, ll.mass, max_inflight=4, max_batch=16) as srv:
+            t1 = time.perf_counter()
+            ids = []
+            for r, gap in zip(reqs, gaps):
+                time.sleep(gap)
+                ids.append(submit(srv, r))
+            results = []
+            for rid in ids:
+                srv.wait(rid)
+                results.append(srv.result(rid)[0])
+            wall = time.perf_counter() - t1
+            st = srv.stats()
+        levels.append({"offered_load": load, "arrival_rate_rps": load * cap_rps, "requests": len(reqs),
+                       "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms, "max_ms": st.max_ms,
+                       "throughput_rps": len(reqs) / wall, "evals_per_s": st.evals / max(st.busy_s, 1e-9),
+                       "batches": st.batches, "failed": st.failed, "peak_hbm_bytes": st.peak_bytes})
+        log(f"load {load}: p50 {st.p50_ms:.1f} ms p99 {st.p99_ms:.1f} ms, {len(reqs) / wall:.2f} req/s, "
+            f"{st.batches} batches")
+        keep = list(zip(reqs, results))
+    # alone == in the mix, on a sample of the last level
+    same = 0
+    sample = keep[:: max(1, len(keep) // 6)][:6]
+    with Server(lines, ll.mass, max_inflight=1, max_batch=1) as srv:
+        for r, mixed in sample:
+            rid = submit(srv, r)
+            srv.wait(rid)
+            same += int(np.array_equal(srv.result(rid)[0], mixed))
+    lines.close()
+    top = levels[-1]
+    res = {"metric": "serving p50/p99 latency (ms) at 90% offered load; throughput", "value": top["p99_ms"],
+           "unit": "ms (p99)", "n_gpus": 1, "steps": args.requests, "warmup": 0,
+           "ms_per_step": top["mean_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"serving_3e8: {ll.n} resident lines, Poisson arrivals, Doppler/Voigt 50/50, "
+                                  "T U[200,3000] K, P log-U[1e-5,100] atm, width log-U[10,12000] cm^-1, "
+                                  "resolution log-U[1e-3,0.1] cm^-1 (<= 1e7 bins), Voigt cutoff 25 cm^-1 mixed, "
+                                  "Doppler 25 HWHM FP64", "closed_loop_capacity_rps": cap_rps},
+           "levels": levels, "alone_equals_mix": f"{same}/{len(sample)} sampled requests bit-identical"}
+    print(json.dumps(res), flush=True)
+
+
+def bench_doppler(args, wl):
+    """Doppler cross-section of 1e5 lines on a 1e6-bin grid (25-HWHM per-line windows), FP64."""
+    from gpu_cross_amd import capi
+    from gpu_cross_amd.manager import Mi355xManager, make_grid
+    from gpu_cross_amd.synth import workload
+    from oracle import pyoracle as po
+
+    ll = workload("doppler_1e5").sorted_by_nu()
+    T = wl["T"]
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    start, end, M = wl["grid"]
+    freq, d = make_grid(start, end, M)
+    m = Mi355xManager(capi.DOPPLER, doppler_widths=wl["doppler_widths"])
+    m.InitializeVectors(M)
+    m.TransferFreq(freq, np.zeros(M), M)
+    m.InitializeConstants(0.0, T, Q, d, ll.mass, 1.0, 296.0)
+    m.TransferVectors(ll.n, ll.elow, ll.nu, ll.aif, ll.gns)
+    for _ in range(max(args.warmup, 1)):
+        m.ExecuteCrossSection(M, ll.n, 0)
+    m.synchronize()
+    m.TransferFreq(freq, np.zeros(M), M)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        m.ExecuteCrossSection(M, ll.n, 0)
+    m.synchronize()
+    wall = time.perf_counter() - t1
+    t = m.last_timing()
+    out = np.zeros(M)
+    m.TransferResults(None, out, M)
+    m.close()
+    out /= args.steps
+    # the oracle with the reference's 100 cm^-1 window on the same lines (full, it is small)
+    t2 = time.perf_counter()
+    c = po.consts(po.DOPPLER, T, Q, d, ll.mass, 1.0, 296.0)
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    ref = po.cross_section(c, ll.elow, ll.nu, ll.aif, ll.gns, None, None, freq, M, 0, cores)
+    cpu_s = time.perf_counter() - t2
+    mask = ref >= 1e-6 * ref.max()
+    rel = np.abs(out[mask] - ref[mask]) / ref[mask]
+    res = {"metric": "Doppler cross-sections/s (FP64)", "value": args.steps / wall, "unit": "xsec/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"doppler_1e5: {ll.n} lines, {M} bins {start}-{end} cm^-1, T={T} K, windows "
+                                  f"{wl['doppler_widths']} HWHM + 1 bin (the reference's 100 cm^-1 window gives "
+                                  "the same grid)", "evals_per_step": t.evals},
+           "precision": {"max_rel_err_vs_oracle_100cm": float(rel.max()), "bins_checked": int(mask.sum())},
+           "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "xsec/s", "cores": cores, "kind": "port",
+                            "sample": "the whole cross-section with the reference's 100 cm^-1 window"}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="table_48", choices=["table_48", "voigt_1e8", "voigt_1e7"])
+    ap.add_argument("--workload", default="table_48",
+                    choices=["table_48", "voigt_1e8", "voigt_1e7", "serving_3e8", "doppler_1e5"])
+    ap.add_argument("--requests", type=int, default=120, help="serving: requests per load level")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -201,6 +351,10 @@ def main():
     wl = WORKLOADS[args.workload]
     if args.workload == "table_48":
         return bench_table(args, wl)
+    if args.workload == "serving_3e8":
+        return bench_serving(args, wl)
+    if args.workload == "doppler_1e5":
+        return bench_doppler(args, wl)
     t0 = time.time()
     ll = workload(args.workload).sorted_by_nu()
     log(f"generated {ll.n} lines in {time.time() - t0:.1f} s")
