@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compensated add
+# HBM bytes per accumulation launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+# same command scaled by a plain-copy calibration (profiles/k_accum_voigt_mixed.table_48.hbm.md)
+PROFILED_TRAFFIC = {"table_48": 7.92e11}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 
@@ -135,7 +138,9 @@ def bench_table(args, wl):
         "ms_per_xsec": ms_step / len(pts),
         "roofline": {"bound": "valu-fp32", "achieved": FLOP_PER_EVAL * evals / (eager_acc * 1e-3) / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": FLOP_PER_EVAL * evals / (eager_acc * 1e-3) / PEAK_FP32, "traffic": None,
+                     "frac": FLOP_PER_EVAL * evals / (eager_acc * 1e-3) / PEAK_FP32,
+                     "traffic": PROFILED_TRAFFIC.get("table_48"),
+                     "traffic_source": "rocprofv3 2*FETCH_SIZE+WRITE_SIZE per launch (copy-calibrated), profiles/",
                      "kernel": "k_accum_voigt_mixed", "kernel_ms": eager_acc, "flop_per_eval": FLOP_PER_EVAL},
         "intensity_kernels_ms": eager_int,
     }

@@ -319,8 +319,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStr
   plan.partial = partial;
   plan.counter = counter;
   plan.nitems = (int)nitems;
-  plan.grid = (int)std::min<long long>(nitems, slots);
-  if (max_grid > 0) plan.grid = std::min(plan.grid, max_grid);
+  plan.grid = persistent ? (int)std::min<long long>(nitems, slots) : (int)nitems;
   return GX_OK;
 }
 

@@ -79,7 +79,8 @@ struct MixedRunner {
   int R = 4;                   // bins per lane (4 or 8)
   int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
   int forced_nseg = 0;         // > 0: fixed segments per block
-  int max_grid = 0;            // > 0: cap on persistent workgroups (device shared with other launches)
+  bool persistent = true;      // false: one workgroup per work item, so the hardware interleaves
+                               // workgroups of launches on other streams (shared device)
   std::vector<MixedTask> h_tasks;
   MixedPlan plan{};
   int init();

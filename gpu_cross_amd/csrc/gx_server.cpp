@@ -10,10 +10,12 @@
 //   3. launches the batch on the slot's own stream, so batches overlap on
 //      the device and new requests start while others are still running.
 // A request only touches the slice of the resident, nu-sorted line list
-// that can reach its grid.  Mixed-precision batches are one persistent
-// launch over all requests of the batch (the table kernels); the work split
-// per request depends on the request's own grid only, so its result is the
-// same alone and in any mix.  FP64 and Doppler requests run the plain
+// that can reach its grid.  Mixed-precision batches are one launch of the
+// table kernel over all requests of the batch, with one workgroup per work
+// item (not persistent), so the hardware interleaves the workgroups of
+// batches on other streams and a lone batch still fills the chip; the work
+// split per request depends on the request's own grid only, so its result
+// is the same alone and in any mix.  FP64 and Doppler requests run the plain
 // per-point kernels.  Slot buffers grow and are reused (no per-batch
 // hipMalloc/hipFree on the critical path).
 #include <hip/hip_runtime.h>
@@ -215,7 +217,7 @@ int launch_batch(gx_server* S, Slot& sl) {
     if ((rc = sl.run.init())) return rc;
     const int slots = sl.run.cus * mixed_occupancy(sl.run.R);
     sl.run.forced_nseg = std::min(16, std::max(1, (4 * slots + nblocks - 1) / nblocks));
-    sl.run.max_grid = std::max(1, (slots + S->o.max_inflight - 1) / S->o.max_inflight);
+    sl.run.persistent = false;
     std::vector<MixedTask> tasks(B);
     for (int k = 0; k < B; ++k)
       tasks[k] = MixedTask{sl.der.m.amp + (size_t)k * sl.der.m.stride, sl.der.m.y + (size_t)k * sl.der.m.stride,
