@@ -136,7 +136,7 @@ def bench_table(args, wl):
                                f"of the whole table per step", "lines": ll.n, "bins": M, "points": len(pts),
                    "evals_per_step": evals},
         "ms_per_xsec": ms_step / len(pts),
-        "roofline": {"bound": "valu-fp32", "achieved": FLOP_PER_EVAL * evals / (eager_acc * 1e-3) / 1e12,
+        "roofline": {"bound": "valu", "achieved": FLOP_PER_EVAL * evals / (eager_acc * 1e-3) / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": FLOP_PER_EVAL * evals / (eager_acc * 1e-3) / PEAK_FP32,
                      "traffic": PROFILED_TRAFFIC.get("table_48"),
@@ -411,7 +411,7 @@ def main():
                                f"cutoff {wl['cutoff']} cm^-1, one (T,P) cross-section per step",
                    "lines": ll.n, "bins": M, "evals_per_step": evals},
         "ms_per_xsec": ms_step,
-        "roofline": {"bound": "valu-fp32", "achieved": FLOP_PER_EVAL * evals / (mixed["acc_ms"] * 1e-3) / 1e12,
+        "roofline": {"bound": "valu", "achieved": FLOP_PER_EVAL * evals / (mixed["acc_ms"] * 1e-3) / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": FLOP_PER_EVAL * evals / (mixed["acc_ms"] * 1e-3) / PEAK_FP32, "traffic": None,
                      "kernel": "k_accum_voigt_mixed", "kernel_ms": mixed["acc_ms"],
