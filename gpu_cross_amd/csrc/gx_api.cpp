@@ -185,13 +185,14 @@ int Derived::reserve(size_t n) {
 
 int Derived::reserve_mixed(size_t n, int npoints) {
   if (n <= mcap && npoints <= mpoints) return GX_OK;
-  hipFree(m.amp); hipFree(m.y);
+  hipFree(m.amp); hipFree(m.y); hipFree(m.g); hipFree(m.core);
   m = MixedLineData{};
   mcap = 0;
   mpoints = 0;
   const size_t c = std::max<size_t>(n, 1), total = c * (size_t)std::max(npoints, 1);
   hipError_t e;
-  if ((e = hipMalloc(&m.amp, total * 4)) || (e = hipMalloc(&m.y, total * 8)))
+  if ((e = hipMalloc(&m.amp, total * 4)) || (e = hipMalloc(&m.y, total * 8)) || (e = hipMalloc(&m.g, total * 8)) ||
+      (e = hipMalloc(&m.core, total * sizeof(int2))))
     return hip_fail(e, "hipMalloc(mixed line data)");
   m.stride = c;
   mcap = c;
@@ -220,7 +221,7 @@ void Derived::release() {
   hipFree(d.S); hipFree(d.g); hipFree(d.y); hipFree(d.ib); hipFree(d.ie);
   d = DerivedF64{};
   cap = 0;
-  hipFree(m.amp); hipFree(m.y);
+  hipFree(m.amp); hipFree(m.y); hipFree(m.g); hipFree(m.core);
   m = MixedLineData{};
   mcap = 0;
   mpoints = 0;
@@ -264,13 +265,14 @@ int MixedRunner::init() {
 }
 
 void MixedRunner::release() {
-  hipFree(d_tasks); hipFree(partial); hipFree(counter);
-  d_tasks = nullptr; partial = nullptr; counter = nullptr;
-  task_cap = 0; partial_cap = 0; cus = 0;
+  hipFree(d_tasks); hipFree(partial); hipFree(counter); hipFree(starts); hipFree(core_part);
+  d_tasks = nullptr; partial = nullptr; counter = nullptr; starts = nullptr; core_part = nullptr;
+  task_cap = 0; partial_cap = 0; cus = 0; starts_cap = 0; core_cap = 0;
   h_tasks.clear();
 }
 
-int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStream_t s) {
+int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const GridView& gv, int nlines,
+                         hipStream_t s) {
   int rc = init();
   if (rc) return rc;
   plan = MixedPlan{};
@@ -320,6 +322,65 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStr
   plan.counter = counter;
   plan.nitems = (int)nitems;
   plan.grid = persistent ? (int)std::min<long long>(nitems, slots) : (int)nitems;
+
+  // Core pass geometry.  A core bin j of a line satisfies |nu - f_j| < (15 - y) alpha
+  // <= 15 dpw nu, so nu < F / (1 - 15 dpw) with F the largest |f| of the chunk, and the
+  // centre bin c is within kmax = 15 dpw F / ((1 - 15 dpw) dnu) + 1/2 (+ margin) of j.
+  plan.nlines = std::max(nlines, 0);
+  if (gv.points > 0 && plan.nlines > 0) {
+    const double d = gv.dfreq;
+    const double F = std::max(std::fabs(gv.start_nu + (double)(gv.start_idx - 1) * d),
+                              std::fabs(gv.start_nu + (double)(gv.start_idx + gv.points + 1) * d));
+    double kb = 0.0;
+    for (int k = 0; k < ntasks; ++k) {
+      const double w = 15.0 * tasks[k].dpwcoef;
+      if (!(w >= 0.0 && w < 0.5)) return fail(GX_INVALID_ARG, "mixed: Doppler width coefficient out of range");
+      kb = std::max(kb, w * F / ((1.0 - w) * d));
+    }
+    if (!(kb < 1e6)) return fail(GX_INVALID_ARG, "mixed: W4 core wider than 1e6 bins; use FP64 for this grid");
+    const int kmax = (int)std::ceil(kb) + 3;
+    // Centre bins per workgroup from the grid alone (a core half-width estimate at
+    // dpw = 1e-5), so that a task's rows, hence its sums, do not depend on the other
+    // tasks of the launch; rows then cost ~1-2 doubles per bin.  Row boundaries sit
+    // at multiples of cpw in absolute bins for the same reason.
+    const double kg = std::min(1e6, 15.0 * 1e-5 * F / d);
+    int cpw = (int)std::min(1024.0, std::max(64.0, std::ceil(2.0 * kg / 64.0) * 64.0));
+    if (core_lds_bytes(cpw + 2 * kmax + 1) == 0) cpw = 64;
+    const int span = cpw + 2 * kmax + 1;
+    if (core_lds_bytes(span) == 0)
+      return fail(GX_INVALID_ARG, "mixed: grid step too fine for the W4 core pass (core wider than 4000 bins); "
+                                  "use FP64 for this grid");
+    const long long lo = (long long)gv.start_idx - kmax - 1;
+    const long long cbase = (lo >= 0 ? lo / cpw : -((-lo + cpw - 1) / cpw)) * cpw;
+    const long long cend = (long long)gv.start_idx + gv.points + kmax + 1;
+    plan.kmax = kmax;
+    plan.cpw = cpw;
+    plan.span = span;
+    plan.cbase = (int)cbase;
+    plan.ncwg = (int)((cend - cbase + cpw - 1) / cpw);
+    plan.ncent = plan.ncwg * cpw;
+    const size_t need_starts = (size_t)plan.ncent + 1;
+    const size_t need_rows = (size_t)ntasks * plan.ncwg * span;
+    if (need_starts > starts_cap || need_rows > core_cap) {
+      if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
+      if (need_starts > starts_cap) {
+        hipFree(starts);
+        starts = nullptr;
+        starts_cap = 0;
+        if ((e = hipMalloc(&starts, need_starts * sizeof(int)))) return hip_fail(e, "hipMalloc(core starts)");
+        starts_cap = need_starts;
+      }
+      if (need_rows > core_cap) {
+        hipFree(core_part);
+        core_part = nullptr;
+        core_cap = 0;
+        if ((e = hipMalloc(&core_part, need_rows * sizeof(double)))) return hip_fail(e, "hipMalloc(core rows)");
+        core_cap = need_rows;
+      }
+    }
+    plan.starts = starts;
+    plan.core_part = core_part;
+  }
   return GX_OK;
 }
 
@@ -350,14 +411,15 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   if (mixed) {
     if ((rc = der.reserve_mixed((size_t)std::max(nlines, 1), 1))) return rc;
     if ((rc = der.set_points(&pc, 1, s))) return rc;
-    MixedTask task{der.m.amp, der.m.y, pc.dpwcoef, intens};
-    if ((rc = mix.prepare(&task, 1, nblocks, s))) return rc;
+    MixedTask task{der.m.amp, der.m.y, der.m.g, der.m.core, pc.dpwcoef, intens};
+    if ((rc = mix.prepare(&task, 1, nblocks, gv, nlines, s))) return rc;
   }
   if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");
   if (mixed) {
     const WindowArgs wa{pc.cutoff, pc.convention, 1};
     if ((e = launch_windows(wa, gv, ls.nu, nlines, der.d.ib, der.d.ie, d_evals, s))) return hip_fail(e, "windows");
-    if ((e = launch_prep_mixed(der.d_pcs, 1, lv, der.m, s))) return hip_fail(e, "mixed prep launch");
+    if ((e = launch_prep_mixed(der.d_pcs, 1, lv, gv, der.d.ib, der.d.ie, der.m, s)))
+      return hip_fail(e, "mixed prep launch");
   } else {
     if ((e = launch_line_prep_f64(pc, gv, lv, der.d, d_evals, s))) return hip_fail(e, "line prep launch");
   }
@@ -402,7 +464,7 @@ struct gx_ctx {
 
 namespace {
 
-constexpr size_t kBytesPerLine = 44 + 32 + 52;   // resident + derived + sort staging
+constexpr size_t kBytesPerLine = 44 + 60 + 52;   // resident + derived (FP64 + mixed) + sort staging
 constexpr size_t kMaxNtrans = size_t(1) << 30;   // keeps Nener within the int of ExecuteCrossSection
 
 int set_device(gx_ctx* c) {

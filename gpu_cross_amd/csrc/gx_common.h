@@ -75,6 +75,10 @@ struct MixedRunner {
   float2* partial = nullptr;
   size_t partial_cap = 0;      // float2 elements
   unsigned* counter = nullptr;
+  int* starts = nullptr;       // core pass: centre-bin index of the sorted lines
+  size_t starts_cap = 0;
+  double* core_part = nullptr; // core pass: per-workgroup bin rows
+  size_t core_cap = 0;
   int cus = 0;
   int R = 4;                   // bins per lane (4 or 8)
   int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
@@ -85,9 +89,10 @@ struct MixedRunner {
   MixedPlan plan{};
   int init();
   int bins_per_block() const { return mixed_bins_per_block(R); }
-  // Upload the task table, size the partials and fix the launch shape.
+  // Upload the task table, size the partials and the core-pass buffers for the
+  // chunk gv of the grid and nlines sorted lines, and fix the launch shape.
   // May synchronise the stream; never call it inside a graph capture.
-  int prepare(const MixedTask* tasks, int ntasks, int nblocks, hipStream_t s);
+  int prepare(const MixedTask* tasks, int ntasks, int nblocks, const GridView& gv, int nlines, hipStream_t s);
   // Enqueue the kernels of the prepared plan (graph-capturable).
   int launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie, hipStream_t s);
   void release();

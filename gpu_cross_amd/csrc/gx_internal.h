@@ -56,11 +56,14 @@ struct DerivedF64 {
 };
 
 // Per-point line data of the mixed-precision path, [npoints][stride].  1/alpha
-// is recomputed in FP64 at staging (one division), and y is kept in FP64, so
-// every W4 region decision is the FP64 path's own.
+// and y stay in FP64 and the W4 core range of every line is fixed here, as an
+// integer bin range found with the FP64 path's own test |nu - f_j|/alpha + y < 15,
+// so every W4 region decision is the FP64 path's own.
 struct MixedLineData {
   float* amp;     // S / alpha (FP32)
   double* y;      // gamma_T / alpha
+  double* g;      // 1 / alpha = 1 / (nu * dpwcoef)
+  int2* core;     // [k0, k1): bins of the window where |x| + y < 15 (regions II-IV)
   size_t stride;  // lines per point
 };
 
@@ -75,6 +78,8 @@ struct WindowArgs {
 struct MixedTask {
   const float* amp;
   const double* y;
+  const double* g;
+  const int2* core;
   double dpwcoef;     // Doppler width per unit nu of this point
   double* out;
 };
@@ -95,6 +100,16 @@ struct MixedPlan {
   unsigned* counter;       // work-queue head, reset by the launch
   int nitems;              // ntasks * nblocks * nseg
   int grid;                // persistent workgroups
+  // W4 core pass (regions II-IV): lines grouped by centre bin c = rint((nu - nu0)/dnu)
+  int nlines;              // lines in nu[]
+  int* starts;             // [ncent + 1]: first line whose centre bin is >= cbase + c
+  int cbase;               // first centre bin that can reach the chunk
+  int ncent;               // centre bins [cbase, cbase + ncent)
+  int kmax;                // bound on |bin - centre bin| inside any core
+  int cpw;                 // centre bins per core workgroup
+  int span;                // bins of one core workgroup's row: cpw + 2 kmax + 1
+  int ncwg;                // core workgroups per task
+  double* core_part;       // [ntasks][ncwg][span] per-workgroup core sums
 };
 
 constexpr int kEvalSlots = 64;       // eval counters (spread to avoid one hot atomic)
@@ -116,13 +131,15 @@ hipError_t launch_accum_f64(const PointConsts& pc, const GridView& gv, const dou
                             const int2* ranges, int nblocks, double* intens, hipStream_t s);
 hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double* nu, int n, int* ib, int* ie,
                           unsigned long long* evals, hipStream_t s);
-hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, MixedLineData out,
-                             hipStream_t s);
+hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, const GridView& gv,
+                             const int* ib, const int* ie, MixedLineData out, hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s);
 // R = bins per lane (4 or 8); a workgroup covers mixed_bins_per_block(R) bins
 hipError_t launch_accum_mixed(const MixedPlan& plan, int R, hipStream_t s);
 int mixed_occupancy(int R);   // resident workgroups per CU of the mixed kernel
+// LDS bytes of a core-pass workgroup with the given row span (0: the row does not fit LDS)
+size_t core_lds_bytes(int span);
 int mixed_bins_per_block(int R);
 
 // bins covered by one workgroup of the FP64 accumulation kernels

@@ -182,9 +182,20 @@ __global__ __launch_bounds__(256) void k_windows(WindowArgs wa, GridView gv, con
 
 // ---------------------------------------------------------------------------
 // Per-point line data of the mixed-precision path: S, 1/alpha and y in FP64
-// exactly as k_line_prep, stored as FP32 (amp = S/alpha).  blockIdx.y = point.
+// exactly as k_line_prep, amp = S/alpha stored as FP32, and the line's W4 core
+// inside its window: the bins j of [ib, ie) with |nu - f_j| g + y < 15, the FP64
+// path's own region-I test (evaluated on the grid array itself).  The set is
+// an interval (|nu - f_j| is V-shaped in j); it is found from a bracketing
+// estimate one bin wider on each side and trimmed with the exact test.
+// blockIdx.y = point.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, LinesView lv,
+__device__ __forceinline__ bool in_core(double nu, double g, double y, const double* freq, int j) {
+#pragma clang fp contract(off)
+  return fabs(nu - freq[j]) * g + y < 15.0;
+}
+
+__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, LinesView lv, GridView gv,
+                                                    const int* __restrict__ IB, const int* __restrict__ IE,
                                                     MixedLineData out) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,9 +211,33 @@ __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restric
   if (pc.convention == 1 && gi == -1) S = a;
   const double g = 1.0 / (nu_if * pc.dpwcoef);
   const double gammaL = (lv.gamma[i] * pc.pressure_scale) * pow(pc.ref_temp / pc.temperature, lv.nexp[i]);
+  const double y = gammaL * g;
+  const int ib = IB[i], ie = IE[i];
+  int k0 = 0, k1 = 0;
+  if (y < 15.0 && ie > ib) {
+    const double half = (15.0 - y) / g;
+    const double lo = floor((nu_if - half - gv.start_nu) / gv.dfreq) - 1.0;
+    const double hi = ceil((nu_if + half - gv.start_nu) / gv.dfreq) + 1.0;
+    int a0 = (int)fmax(lo, (double)ib), z = (int)fmin(hi, (double)ie);
+    while (a0 < z && !in_core(nu_if, g, y, gv.freq, a0)) ++a0;
+    while (z > a0 && !in_core(nu_if, g, y, gv.freq, z - 1)) --z;
+    if (z > a0) {
+      k0 = a0;
+      k1 = z;
+    }
+  }
   const size_t o = (size_t)p * out.stride + i;
   out.amp[o] = (float)(S * g);
-  out.y[o] = gammaL * g;
+  out.y[o] = y;
+  out.g[o] = g;
+  out.core[o] = make_int2(k0, k1);
+}
+
+// Centre bin of a line on the grid (nearest node), clamped far inside int range.
+__device__ __forceinline__ int centre_bin(double nu_if, const GridView& gv) {
+#pragma clang fp contract(off)
+  const double c = rint((nu_if - gv.start_nu) / gv.dfreq);
+  return (int)fmin(fmax(c, -1073741824.0), 1073741824.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -297,11 +332,10 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
 //   r      (nu - f_{c}) / alpha      (|r| <= e/2)
 //   e      dfreq / alpha
 // so x_j = r - (j - c) e with an exact integer factor and the FP32 offset
-// error stays relative to x itself.  Region I of W4 runs in FP32 as
-//   (A1 + A2 x^2) / (x^2 (x^2 + R) + P2),
-//   A1 = amp c y (0.5+y^2), A2 = amp c y, R = 2y^2 - 1, P2 = (0.5+y^2)^2,
-// and bins with |x| + y < 15 (the core, found per line in FP64 as an integer
-// range so the region choice equals the FP64 path's) go to the FP64 W4.
+// error stays relative to x itself.  Region I of W4 runs in FP32 (region1()),
+// and bins with |x| + y < 15 (the core, fixed per line by k_prep_mixed as an
+// integer range, so the region choice equals the FP64 path's) are left to the
+// core pass (k_core_mixed), which evaluates regions II-IV line by line.
 // Window and core tests are scalar per (wave, line): a wave whose bins all
 // lie inside the window and outside the core runs the branch-free loop.
 // Per-bin sums: plain FP32 over 32 lines, then a Kahan step into (sum, comp).
@@ -346,8 +380,6 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
   constexpr int kWaveBins = MixedShape<R>::kWaveBins;
   constexpr int kBinsMixed = MixedShape<R>::kBins;
   __shared__ StagedLine s_l[kTile];
-  __shared__ double s_nu[kTile], s_g[kTile], s_y[kTile];
-  __shared__ float s_amp[kTile];
   __shared__ int s_item;
 
   const int t = threadIdx.x;
@@ -392,12 +424,15 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
         const double nu = v0 == 0.0 ? 1e-6 : v0;
-        const double g = 1.0 / (nu * tk.dpwcoef), y = tk.y[i];     // as k_line_prep: same bits
+        const double g = tk.g[i], y = tk.y[i];                      // k_prep_mixed: FP64 path's bits
         const double cj = rint((nu - gv.start_nu) / gv.dfreq);      // centre bin
         const double fc = gv.start_nu + cj * gv.dfreq;
         StagedLine L;
         L.ib = P.ib[i];
         L.ie = P.ie[i];
+        const int2 core = tk.core[i];
+        L.k0 = core.x;
+        L.k1 = core.y;
         const float amp = tk.amp[i];
         // Power-of-two rescaling x' = sigma x keeps x'^4 far from FP32 overflow for lines
         // whose window spans huge x (tiny nu => huge 1/alpha); region I is invariant under
@@ -417,30 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
         L.a = (float)(ac * sg2);
         L.b = (float)(ac * sg4);
         L.cf = (float)(cj - (double)j0);
-        // core bins: |nu - f_j| g + y < 15, as an exact integer range [k0, k1)
-        int k0 = 0, k1 = 0;
-        if (y < 15.0 && L.ie > L.ib) {
-          const double half = (15.0 - y) / g;
-          int a = (int)floor((nu - half - gv.start_nu) / gv.dfreq) - 1;
-          int z = (int)ceil((nu + half - gv.start_nu) / gv.dfreq) + 1;
-          a = max(a, L.ib);
-          z = min(z, L.ie);
-          auto core = [&](int jj) {
-            const double fj = gv.start_nu + (double)jj * gv.dfreq;
-            return fabs(nu - fj) * g + y < 15.0;
-          };
-          while (a < z && !core(a)) ++a;
-          while (z > a && !core(z - 1)) --z;
-          k0 = a;
-          k1 = z;
-        }
-        L.k0 = k0;
-        L.k1 = k1;
         s_l[t] = L;
-        s_nu[t] = nu;
-        s_g[t] = g;
-        s_y[t] = y;
-        s_amp[t] = amp;
       }
       __syncthreads();
       const int cnt = min(kTile, hi - base);
@@ -514,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
           }
         }
       }
-      // edge lines: per-lane window and core tests; core bins in FP64
+      // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
 #pragma unroll
       for (int m = 0; m < kTile / 64; ++m) {
         unsigned long long bits = edge_m[m];
@@ -527,14 +539,8 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
 #pragma unroll
           for (int q = 0; q < R; ++q) {
             const int jj = w0 + lane + 64 * q;
-            if (jj >= I.x && jj < I.y) {
-              if (jj >= I.z && jj < I.w) {
-                const double x = fabs(s_nu[k] - gv.freq[jj]) * s_g[k];
-                part[q] += s_amp[k] * w4_core_mixed(x, s_y[k]);
-              } else {
-                part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
-              }
-            }
+            if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
+              part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
           }
           if (++since_flush == 32) {
             since_flush = 0;
@@ -577,6 +583,162 @@ __global__ void k_reduce_segments(MixedPlan P, int kBinsMixed) {
     const float2 v = P.partial[(item0 + sg) * kBinsMixed + m];
     acc += (double)v.x - (double)v.y;
   }
+  P.tasks[task].out[j] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// W4 core pass: regions II-IV of every (line, bin) pair with bin in the line's
+// core [k0, k1), added to the grids after the region-I kernel.
+//
+// Lines are taken in nu order, 64 per wave, one line per lane; the wave walks
+// the offsets k = bin - centre bin together, so all lanes sit at about the
+// same x and take the same W4 region (the branches stay coherent).  A line's
+// value at offset k belongs to bin c + k, c its centre bin.  Lanes with equal
+// c form contiguous segments (nu is sorted); a segment of one lane adds its
+// value straight into the workgroup's row of bin sums, a longer one is first
+// summed across lanes by a fixed DPP tree.  A workgroup (one wave) owns the
+// lines of P.cpw consecutive centre bins and a row of FP64 sums over
+// the bins they can reach; k_core_gather adds the rows covering a bin in row
+// order.  Every sum is in a fixed order: the grids are deterministic.
+// ---------------------------------------------------------------------------
+
+// first line whose centre bin is >= cbase + c, for c in [0, ncent]
+__global__ void k_centre_starts(GridView gv, const double* __restrict__ nu, int n, int cbase, int ncent,
+                                int* __restrict__ starts) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncent) return;
+  const int target = cbase + c;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    const double v = nu[mid];
+    if (centre_bin(v == 0.0 ? 1e-6 : v, gv) < target) lo = mid + 1; else hi = mid;
+  }
+  starts[c] = lo;
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_take(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Inclusive scan within rows of 16 (row_shr 1, 2, 4, 8), then the row totals
+// carried by row_bcast 15 and 31: lane 63 ends with the sum of all 64 lanes,
+// always added in the same order.
+__device__ __forceinline__ float wave_sum_lane63(float v) {
+  v += dpp_take<0x111, 0xf>(v);
+  v += dpp_take<0x112, 0xf>(v);
+  v += dpp_take<0x114, 0xf>(v);
+  v += dpp_take<0x118, 0xf>(v);
+  v += dpp_take<0x142, 0xa>(v);
+  v += dpp_take<0x143, 0xc>(v);
+  return v;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// kFreqLds: grid values of the row staged in LDS next to the sums (else read from HBM).
+template <bool kFreqLds>
+__global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
+  extern __shared__ double s_core[];
+  const int lane = threadIdx.x;
+  const int w = blockIdx.x;
+  const MixedTask tk = P.tasks[blockIdx.y];
+  const GridView gv = P.gv;
+  const int span = P.span;
+  const int c0 = P.cbase + w * P.cpw;
+  const int base = c0 - P.kmax;                 // row entry i is bin base + i
+  double* acc = s_core;
+  const double* fq = kFreqLds ? s_core + span : gv.freq;
+  const int fq_off = kFreqLds ? base : 0;
+  const int s0 = gv.start_idx, s1 = gv.start_idx + gv.points;
+  for (int i = lane; i < span; i += 64) {
+    acc[i] = 0.0;
+    if (kFreqLds) {
+      const int jj = base + i;
+      s_core[span + i] = (jj >= s0 && jj < s1) ? gv.freq[jj] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int l0 = P.starts[w * P.cpw];
+  const int l1 = P.starts[min((w + 1) * P.cpw, P.ncent)];
+  for (int chunk = l0; chunk < l1; chunk += 64) {
+    const int l = chunk + lane;
+    const bool act = l < l1;
+    double nu = 1e-6, g = 0.0, y = 15.0;
+    float amp = 0.0f;
+    int2 core = make_int2(0, 0);
+    int cj = 0x7fffffff - 64 + lane;            // inactive lanes: singleton segments out of reach
+    if (act) {
+      const double v0 = P.nu[l];
+      nu = v0 == 0.0 ? 1e-6 : v0;
+      g = tk.g[l];
+      y = tk.y[l];
+      amp = tk.amp[l];
+      core = tk.core[l];
+      cj = centre_bin(nu, gv);
+    }
+    const bool has = act && core.y > core.x;
+    if (!__ballot(has)) continue;
+    const int kmin = max(wave_min(has ? core.x - cj : 0x3fffffff), -P.kmax);
+    const int kend = min(wave_max(has ? core.y - cj : -0x3fffffff), P.kmax + 1);
+    const int cprev = __shfl_up(cj, 1), cnext = __shfl_down(cj, 1);
+    const bool head = lane == 0 || cprev != cj;
+    const bool tail = lane == 63 || cnext != cj;
+    const bool single = head && tail;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long multi = __ballot(head && !tail);
+    for (int k = kmin; k < kend; ++k) {
+      const int j = cj + k;
+      float v = 0.0f;
+      const bool on = has && j >= core.x && j < core.y && (unsigned)(j - base) < (unsigned)span;
+      if (on) {
+        const double x = fabs(nu - fq[j - fq_off]) * g;
+        v = amp * w4_core_mixed(x, y);
+      }
+      if (single && on) acc[j - base] += (double)v;
+      const unsigned long long onm = __ballot(on);
+      unsigned long long h = multi & (onm ? ~0ull : 0ull);
+      while (h) {
+        const int a = __builtin_ctzll(h);
+        h &= h - 1;
+        const unsigned long long after = heads & ~((2ull << a) - 1ull);   // heads after lane a
+        const int e = after ? __builtin_ctzll(after) : 64;
+        const unsigned long long seg = (e == 64 ? ~0ull : (1ull << e) - 1ull) & ~((1ull << a) - 1ull);
+        if (!(onm & seg)) continue;
+        const float tot = wave_sum_lane63(lane >= a && lane < e ? v : 0.0f);
+        const int ia = __builtin_amdgcn_readlane(cj, a) + k - base;
+        if (lane == 63 && (unsigned)ia < (unsigned)span) acc[ia] += (double)tot;
+      }
+    }
+  }
+  __syncthreads();
+  double* row = P.core_part + ((size_t)blockIdx.y * P.ncwg + w) * span;
+  for (int i = lane; i < span; i += 64) row[i] = acc[i];
+}
+
+// out[j] += the core rows covering bin j, in row order
+__global__ void k_core_gather(MixedPlan P) {
+  const int j = P.gv.start_idx + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P.gv.start_idx + P.gv.points) return;
+  const int task = blockIdx.y;
+  const int r = j - P.cbase + P.kmax;                       // >= 0
+  const int whi = min(r / P.cpw, P.ncwg - 1);
+  const int t = r - P.span;                                 // rows with w * cpw > t
+  const int wlo = t < 0 ? 0 : t / P.cpw + 1;
+  const double* rows = P.core_part + (size_t)task * P.ncwg * P.span;
+  double acc = 0.0;
+  for (int w = wlo; w <= whi; ++w) acc += rows[(size_t)w * P.span + (r - w * P.cpw)];
   P.tasks[task].out[j] += acc;
 }
 
@@ -639,11 +801,18 @@ hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double
   return hipGetLastError();
 }
 
-hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, MixedLineData out,
-                             hipStream_t s) {
+hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, const GridView& gv,
+                             const int* ib, const int* ie, MixedLineData out, hipStream_t s) {
   if (lv.n <= 0 || npts <= 0) return hipSuccess;
-  k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, out);
+  k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, gv, ib, ie, out);
   return hipGetLastError();
+}
+
+size_t core_lds_bytes(int span) {
+  if (span <= 0) return 0;
+  if ((size_t)span * 16 <= 65536) return (size_t)span * 16;   // row sums + grid values
+  if ((size_t)span * 8 <= 65536) return (size_t)span * 8;     // row sums; grid values from HBM
+  return 0;
 }
 
 hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s) {
@@ -669,10 +838,25 @@ hipError_t launch_accum_mixed(const MixedPlan& p, int R, hipStream_t s) {
   else
     k_accum_voigt_mixed<8><<<p.grid, kBlock, 0, s>>>(p);
   hipError_t e = hipGetLastError();
-  if (e || p.nseg == 1) return e;
-  const int bpb = mixed_bins_per_block(R);
-  const long long n = (long long)p.nblocks * bpb * p.ntasks;
-  k_reduce_segments<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, bpb);
+  if (e) return e;
+  if (p.nseg > 1) {
+    const int bpb = mixed_bins_per_block(R);
+    const long long n = (long long)p.nblocks * bpb * p.ntasks;
+    k_reduce_segments<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, bpb);
+    if ((e = hipGetLastError())) return e;
+  }
+  // W4 core pass
+  if (p.nlines <= 0 || p.ncwg <= 0 || p.gv.points <= 0) return hipSuccess;
+  k_centre_starts<<<(p.ncent + 1 + 255) / 256, 256, 0, s>>>(p.gv, p.nu, p.nlines, p.cbase, p.ncent, p.starts);
+  if ((e = hipGetLastError())) return e;
+  const size_t lds = core_lds_bytes(p.span);
+  if (lds == 0) return hipErrorInvalidValue;
+  if (lds == (size_t)p.span * 16)
+    k_core_mixed<true><<<dim3(p.ncwg, p.ntasks), 64, lds, s>>>(p);
+  else
+    k_core_mixed<false><<<dim3(p.ncwg, p.ntasks), 64, lds, s>>>(p);
+  if ((e = hipGetLastError())) return e;
+  k_core_gather<<<dim3((p.gv.points + 255) / 256, p.ntasks), 256, 0, s>>>(p);
   return hipGetLastError();
 }
 

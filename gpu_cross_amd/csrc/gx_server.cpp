@@ -139,9 +139,10 @@ size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& b
   const size_t M = (size_t)r.npoints, B = batch.size();
   size_t need = std::max(sl.freq_cap, M) * 8 + std::max(sl.out_cap, M * B) * 8 + std::max(sl.win_cap, nlines) * 8;
   if (mixed_req(r)) {
-    need += std::max(sl.der.mcap * (size_t)sl.der.mpoints, nlines * B) * 12;
+    need += std::max(sl.der.mcap * (size_t)sl.der.mpoints, nlines * B) * 28;
     const size_t nb = (M + 1023) / 1024 + 1;
     need += nb * B * 16 * 1024 * sizeof(float2) / 4;   // partials, upper estimate
+    need += (M + 4096) * B * 4 * sizeof(double);       // core-pass rows, upper estimate
   } else {
     need += std::max(sl.der.cap, nlines) * 32;
   }
@@ -221,8 +222,9 @@ int launch_batch(gx_server* S, Slot& sl) {
     std::vector<MixedTask> tasks(B);
     for (int k = 0; k < B; ++k)
       tasks[k] = MixedTask{sl.der.m.amp + (size_t)k * sl.der.m.stride, sl.der.m.y + (size_t)k * sl.der.m.stride,
+                           sl.der.m.g + (size_t)k * sl.der.m.stride, sl.der.m.core + (size_t)k * sl.der.m.stride,
                            sl.h_pcs[k].dpwcoef, sl.out + (size_t)k * M};
-    if ((rc = sl.run.prepare(tasks.data(), B, nblocks, s))) return rc;
+    if ((rc = sl.run.prepare(tasks.data(), B, nblocks, gv, n, s))) return rc;
   }
   if ((e = hipEventRecord(sl.ev_start, s))) return hip_fail(e, "server: event");
   if ((e = launch_grid(sl.freq, M, r0.start_nu, d, s))) return hip_fail(e, "server: grid");
@@ -232,7 +234,7 @@ int launch_batch(gx_server* S, Slot& sl) {
     const WindowArgs wa{pc0.cutoff, pc0.convention, 1};
     if (mixed) {
       if ((e = launch_windows(wa, gv, lv.nu, n, sl.ib, sl.ie, sl.evals, s))) return hip_fail(e, "server: windows");
-      if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, sl.der.m, s))) return hip_fail(e, "server: prep");
+      if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, gv, sl.ib, sl.ie, sl.der.m, s))) return hip_fail(e, "server: prep");
       if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
       if ((rc = sl.run.launch(gv, lv.nu, sl.rng.r, sl.ib, sl.ie, s))) return rc;
     } else {

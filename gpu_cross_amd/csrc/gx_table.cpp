@@ -123,7 +123,7 @@ int enqueue(gx_table* t, bool timed) {
       const Group& g = t->groups[l.group];
       // points of one launch are contiguous in the constants table (grouped at create)
       const PointConsts* pc0 = t->der.d_pcs + g.first_const + l.first;
-      if ((e = launch_prep_mixed(pc0, l.count, lv, t->der.m, s))) return hip_fail(e, "prep");
+      if ((e = launch_prep_mixed(pc0, l.count, lv, gv, g.ib, g.ie, t->der.m, s))) return hip_fail(e, "prep");
       if (first && timed && (e = hipEventRecord(t->ev[1], s))) return hip_fail(e, "event");
       first = false;
       int rc = l.run.launch(gv, lv.nu, g.rng.r, g.ib, g.ie, s);
@@ -155,10 +155,11 @@ int prepare_launches(gx_table* t) {
     for (int k = 0; k < l.count; ++k) {
       const int p = g.points[l.first + k];
       const size_t off = (size_t)k * t->der.m.stride;
-      tasks[k] = MixedTask{t->der.m.amp + off, t->der.m.y + off, t->pcs[p].dpwcoef, t->d_out + (size_t)p * t->M};
+      tasks[k] = MixedTask{t->der.m.amp + off, t->der.m.y + off, t->der.m.g + off, t->der.m.core + off,
+                           t->pcs[p].dpwcoef, t->d_out + (size_t)p * t->M};
     }
     const int nb = (t->M + mixed_bins_per_block(t->R) - 1) / mixed_bins_per_block(t->R);
-    int rc = l.run.prepare(tasks.data(), l.count, nb, t->stream);
+    int rc = l.run.prepare(tasks.data(), l.count, nb, grid_view(t), (int)t->L->ls.n, t->stream);
     if (rc) return rc;
   }
   for (Group& g : t->groups) {
@@ -365,7 +366,8 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
     size_t free_b = 0, total_b = 0;
     if ((e = hipMemGetInfo(&free_b, &total_b))) return bail(hip_fail(e, "hipMemGetInfo"));
     const size_t headroom = size_t(2) << 30;
-    const size_t per_point = 12 * std::max<size_t>(n, 1) + (size_t(64) << 10);
+    // amp, y, 1/alpha, core range per line, and the core pass's rows (~3 doubles per bin)
+    const size_t per_point = 28 * std::max<size_t>(n, 1) + 32 * (size_t)t->M + (size_t(64) << 10);
     const size_t avail = free_b > headroom ? free_b - headroom : 0;
     t->batch = (int)std::min<size_t>((size_t)npts, avail / per_point);
     if (t->batch < 1) return bail(fail(GX_OUT_OF_MEMORY, "table_create: not enough free HBM for one point"));
