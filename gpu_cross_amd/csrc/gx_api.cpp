@@ -341,10 +341,10 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     const int kmax = (int)std::ceil(kb) + 3;
     // Centre bins per workgroup from the grid alone (a core half-width estimate at
     // dpw = 1e-5), so that a task's rows, hence its sums, do not depend on the other
-    // tasks of the launch; rows then cost ~1-2 doubles per bin.  Row boundaries sit
-    // at multiples of cpw in absolute bins for the same reason.
+    // tasks of the launch.  Row boundaries sit at multiples of cpw in absolute bins
+    // for the same reason.
     const double kg = std::min(1e6, 15.0 * 1e-5 * F / d);
-    int cpw = (int)std::min(1024.0, std::max(64.0, std::ceil(2.0 * kg / 64.0) * 64.0));
+    int cpw = (int)std::min(256.0, std::max(64.0, std::ceil(2.0 * kg / 64.0) * 64.0));
     if (core_lds_bytes(cpw + 2 * kmax + 1) == 0) cpw = 64;
     const int span = cpw + 2 * kmax + 1;
     if (core_lds_bytes(span) == 0)

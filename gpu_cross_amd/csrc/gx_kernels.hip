@@ -706,7 +706,7 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
         const double x = fabs(nu - fq[j - fq_off]) * g;
         v = amp * w4_core_mixed(x, y);
       }
-      if (single && on) acc[j - base] += (double)v;
+      if (single && on) atomicAdd(&acc[j - base], (double)v);   // one lane per address: a plain add, no wait
       const unsigned long long onm = __ballot(on);
       unsigned long long h = multi & (onm ? ~0ull : 0ull);
       while (h) {
@@ -718,7 +718,7 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
         if (!(onm & seg)) continue;
         const float tot = wave_sum_lane63(lane >= a && lane < e ? v : 0.0f);
         const int ia = __builtin_amdgcn_readlane(cj, a) + k - base;
-        if (lane == 63 && (unsigned)ia < (unsigned)span) acc[ia] += (double)tot;
+        if (lane == 63 && (unsigned)ia < (unsigned)span) atomicAdd(&acc[ia], (double)tot);
       }
     }
   }

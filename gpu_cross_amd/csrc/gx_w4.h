@@ -94,7 +94,7 @@ __device__ __forceinline__ double w4(double x, double y) {
 // FP64 t P(u)/Q(u) term, whose real part cancels in FP32 near |x| ~ 5, and
 // takes exp(u) = e^(y^2-x^2)(cos 2xy - i sin 2xy) from the FP32 hardware
 // exp/cos (2.7e-6 max).  The region is decided in FP64 by the caller's x, y.
-__device__ __noinline__ float w4_core_mixed(double xd, double yd) {
+__device__ __forceinline__ float w4_core_mixed(double xd, double yd) {
   const double sd = xd + yd;
   const float x = (float)xd, y = (float)yd;
   if (sd >= 5.5) {  // region II: t (1.410474 + 0.5641896 u) / (0.75 + u (3 + u))
