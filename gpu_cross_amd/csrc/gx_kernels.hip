@@ -350,9 +350,17 @@ struct MixedShape {
 
 struct alignas(16) StagedLine {
   int ib, ie, k0, k1;         // window and core ranges (absolute bins)
-  float r, e, h, k;           // offset operands; h = y^2 - 1/2, k = 2 y^2 (scaled)
-  float a, b, pad, cf;        // a = amp c y, b = a (scaled); cf: centre bin relative to the block origin
+  float e, h, k, a;           // e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
+  float b, r, cf, anc;        // b = a (scaled); r = x at the centre bin; cf: centre bin relative
+                              // to the block origin; anc = 1: anchored offsets usable
 };
+
+// Anchored offsets: x at bin jb + o is A - o e, A = x(jb) rounded once from FP64 and o an
+// exact small integer, one FMA per bin.  With jb the middle of each 64-bin slice, |o| <= 32,
+// the error of x is eps (|x| + |A|) <= eps (2|x| + 32 e), and since region I has
+// |x| + y >= 15 the relative error of the profile stays below eps (1 + 3e); the form is
+// used for lines with e = dnu/alpha <= 2 (7 eps), the exact-integer form elsewhere.
+constexpr double kAnchorMaxE = 2.0;
 
 __device__ __forceinline__ void kahan_step(float& sum, float& comp, float& part) {
   const float yk = part - comp;
@@ -376,14 +384,16 @@ __device__ __forceinline__ float region1(float x, float h, float k, float a, flo
 __global__ void k_reset_counter(unsigned* c) { *c = 0u; }
 
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_accum_voigt_mixed(MixedPlan P) {
   constexpr int kWaveBins = MixedShape<R>::kWaveBins;
   constexpr int kBinsMixed = MixedShape<R>::kBins;
   __shared__ StagedLine s_l[kTile];
+  __shared__ float4 s_anc[kTile * R];              // per line: x at the middle of each 64-bin slice
   __shared__ int s_item;
 
   const int t = threadIdx.x;
   const int lane = t & 63;
+  const float no = (float)(32 - lane);             // -(offset of this lane from its slice middle)
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const GridView gv = P.gv;
   const int jend = gv.start_idx + gv.points;
@@ -443,7 +453,6 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
         const float e = (float)(gv.dfreq * g * sg);
         L.r = (float)((nu - fc) * g * sg);
         L.e = e;
-        L.pad = 0.0f;
         // scaled: u' = x'^2 + (y^2 - 1/2) s^2, den' = u'^2 + 2 y^2 s^4, num' = a s^2 (u' + s^2)
         const double y2 = y * y;
         const double ac = (double)amp * 0.5641896 * y;
@@ -452,7 +461,19 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
         L.a = (float)(ac * sg2);
         L.b = (float)(ac * sg4);
         L.cf = (float)(cj - (double)j0);
+        L.anc = gv.dfreq * g <= kAnchorMaxE ? 1.0f : 0.0f;
         s_l[t] = L;
+        if (L.anc != 0.0f) {
+          // slice n = w R + q (wave w, slice q) has its middle at bin j0 + 64 n + 32
+          const double ed = gv.dfreq * g * sg;
+          const double x32 = (nu - fc) * g * sg + (cj - (double)(j0 + 32)) * ed;
+#pragma unroll 1
+          for (int n4 = 0; n4 < R; ++n4) {
+            const double b0 = x32 - (double)(256 * n4) * ed;
+            s_anc[t * R + n4] = make_float4((float)b0, (float)(b0 - 64.0 * ed), (float)(b0 - 128.0 * ed),
+                                            (float)(b0 - 192.0 * ed));
+          }
+        }
       }
       __syncthreads();
       const int cnt = min(kTile, hi - base);
@@ -460,11 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
       //   fast  -- window covers all of this wave's bins, no W4 core among them
       //   edge  -- touches the wave otherwise (partial window or core bins)
       // and lines that miss the wave entirely are dropped.
-      unsigned long long fast_m[kTile / 64], edge_m[kTile / 64];
+      unsigned long long fasta_m[kTile / 64], fastx_m[kTile / 64], edge_m[kTile / 64];
 #pragma unroll
       for (int m = 0; m < kTile / 64; ++m) {
         const int k = m * 64 + lane;
-        bool f = false, ed = false;
+        bool f = false, ed = false, an = false;
         if (k < cnt) {
           const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
           const bool touch = I.y > w0 && I.x < w1;
@@ -472,59 +493,85 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
           const bool core = I.w > I.z && I.w > w0 && I.z < w1;
           f = full && !core;
           ed = touch && !f;
+          an = s_l[k].anc != 0.0f;
         }
-        fast_m[m] = __ballot(f);
+        fasta_m[m] = __ballot(f && an);
+        fastx_m[m] = __ballot(f && !an);
         edge_m[m] = __ballot(ed);
       }
-      // fast lines: branch-free FP32 region I for all R bins of every lane.  Two records
-      // are in flight (ping-pong): the LDS read of one line overlaps the other's math.
+      auto flush = [&]() {
+        if (++since_flush == 32) {
+          since_flush = 0;
 #pragma unroll
-      for (int m = 0; m < kTile / 64; ++m) {
-        unsigned long long bits = fast_m[m];
-        if (!bits) continue;
+          for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+        }
+      };
+      // Fast lines: branch-free FP32 region I for all R bins of every lane.  Two records
+      // are in flight (ping-pong): the LDS read of one line overlaps the other's math.
+      auto sweep = [&](unsigned long long bits, int m, auto load, auto eval) {
+        if (!bits) return;
         auto next = [&]() {
           const int k = m * 64 + __builtin_ctzll(bits);
           bits &= bits - 1;
           return k;
         };
-        auto eval = [&](const float4& F, const float4& G) {
-#pragma unroll
-          for (int q = 0; q < R; ++q)
-            part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
-          if (++since_flush == 32) {
-            since_flush = 0;
-#pragma unroll
-            for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
-          }
-        };
-        int ka = next();
-        float4 FA = *reinterpret_cast<const float4*>(&s_l[ka].r);
-        float4 GA = *reinterpret_cast<const float4*>(&s_l[ka].a);
-        float4 FB, GB;
+        auto A = load(next());
+        decltype(A) B;
         bool hb = bits != 0;
-        if (hb) {
-          const int kb = next();
-          FB = *reinterpret_cast<const float4*>(&s_l[kb].r);
-          GB = *reinterpret_cast<const float4*>(&s_l[kb].a);
-        }
+        if (hb) B = load(next());
         for (;;) {
-          eval(FA, GA);
+          eval(A);
           if (!hb) break;
           const bool ha = bits != 0;
-          if (ha) {
-            ka = next();
-            FA = *reinterpret_cast<const float4*>(&s_l[ka].r);
-            GA = *reinterpret_cast<const float4*>(&s_l[ka].a);
-          }
-          eval(FB, GB);
+          if (ha) A = load(next());
+          eval(B);
           if (!ha) break;
           hb = bits != 0;
-          if (hb) {
-            const int kb = next();
-            FB = *reinterpret_cast<const float4*>(&s_l[kb].r);
-            GB = *reinterpret_cast<const float4*>(&s_l[kb].a);
-          }
+          if (hb) B = load(next());
         }
+      };
+      struct RecA {
+        float4 E;                 // e, h, k, a
+        float b;
+        float4 H[R / 4];          // x at this wave's slice middles
+      };
+      struct RecX {
+        float4 E, B;              // (e, h, k, a), (b, r, cf, -)
+      };
+      auto load_a = [&](int k) {
+        RecA r;
+        r.E = *reinterpret_cast<const float4*>(&s_l[k].e);
+        r.b = s_l[k].b;
+#pragma unroll
+        for (int qg = 0; qg < R / 4; ++qg) r.H[qg] = s_anc[k * R + wave * (R / 4) + qg];
+        return r;
+      };
+      auto eval_a = [&](const RecA& r) {
+#pragma unroll
+        for (int qg = 0; qg < R / 4; ++qg) {
+          const float hq[4] = {r.H[qg].x, r.H[qg].y, r.H[qg].z, r.H[qg].w};
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq)
+            part[4 * qg + qq] += region1(__builtin_fmaf(no, r.E.x, hq[qq]), r.E.y, r.E.z, r.E.w, r.b);
+        }
+        flush();
+      };
+      auto load_x = [&](int k) {
+        RecX r;
+        r.E = *reinterpret_cast<const float4*>(&s_l[k].e);
+        r.B = *reinterpret_cast<const float4*>(&s_l[k].b);
+        return r;
+      };
+      auto eval_x = [&](const RecX& r) {
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          part[q] += region1(__builtin_fmaf(r.B.z - lanef_q[q], r.E.x, r.B.y), r.E.y, r.E.z, r.E.w, r.B.x);
+        flush();
+      };
+#pragma unroll
+      for (int m = 0; m < kTile / 64; ++m) {
+        sweep(fasta_m[m], m, load_a, eval_a);
+        sweep(fastx_m[m], m, load_x, eval_x);
       }
       // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
 #pragma unroll
@@ -534,19 +581,15 @@ __global__ __launch_bounds__(kBlock) void k_accum_voigt_mixed(MixedPlan P) {
           const int k = m * 64 + __builtin_ctzll(bits);
           bits &= bits - 1;
           const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
-          const float4 F = *reinterpret_cast<const float4*>(&s_l[k].r);
-          const float4 G = *reinterpret_cast<const float4*>(&s_l[k].a);
+          const float4 E = *reinterpret_cast<const float4*>(&s_l[k].e);
+          const float4 B = *reinterpret_cast<const float4*>(&s_l[k].b);
 #pragma unroll
           for (int q = 0; q < R; ++q) {
             const int jj = w0 + lane + 64 * q;
             if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
-              part[q] += region1(__builtin_fmaf(G.w - lanef_q[q], F.y, F.x), F.z, F.w, G.x, G.y);
+              part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
           }
-          if (++since_flush == 32) {
-            since_flush = 0;
-#pragma unroll
-            for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
-          }
+          flush();
         }
       }
       __syncthreads();
