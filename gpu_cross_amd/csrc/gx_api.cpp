@@ -295,7 +295,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     if ((e = hipMemcpy(d_tasks, h_tasks.data(), sizeof(MixedTask) * (size_t)ntasks, hipMemcpyHostToDevice)))
       return hip_fail(e, "mixed: task upload");
   }
-  const int slots = cus * mixed_occupancy(R);
+  const int slots = cus * mixed_occupancy(R) * kWavesPerGroup;   // resident waves
   const long long base = (long long)ntasks * nblocks;
   int nseg = forced_nseg;
   if (nseg <= 0) {
@@ -321,7 +321,9 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
   plan.partial = partial;
   plan.counter = counter;
   plan.nitems = (int)nitems;
-  plan.grid = persistent ? (int)std::min<long long>(nitems, slots) : (int)nitems;
+  const long long waves = persistent ? std::min<long long>(nitems, slots) : nitems;
+  plan.grid = (int)((waves + kWavesPerGroup - 1) / kWavesPerGroup);
+  plan.queue = persistent ? 1 : 0;
 
   // Core pass geometry.  A core bin j of a line satisfies |nu - f_j| < (15 - y) alpha
   // <= 15 dpw nu, so nu < F / (1 - 15 dpw) with F the largest |f| of the chunk, and the

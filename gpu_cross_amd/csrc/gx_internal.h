@@ -99,7 +99,8 @@ struct MixedPlan {
   float2* partial;         // [nitems][bins per block] when nseg > 1
   unsigned* counter;       // work-queue head, reset by the launch
   int nitems;              // ntasks * nblocks * nseg
-  int grid;                // persistent workgroups
+  int grid;                // workgroups launched (kBlock / 64 waves each)
+  int queue;               // 1: waves take items from the counter until none is left; 0: one item per wave
   // W4 core pass (regions II-IV): lines grouped by centre bin c = rint((nu - nu0)/dnu)
   int nlines;              // lines in nu[]
   int* starts;             // [ncent + 1]: first line whose centre bin is >= cbase + c
@@ -114,7 +115,8 @@ struct MixedPlan {
 
 constexpr int kEvalSlots = 64;       // eval counters (spread to avoid one hot atomic)
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
-constexpr int kTile = 256;           // lines staged in LDS per pass
+constexpr int kTile = 256;           // lines staged in LDS per pass (FP64 kernels)
+constexpr int kWavesPerGroup = kBlock / 64;
 
 // ---- launchers (gx_kernels.hip) ----
 hipError_t launch_iota(uint32_t* idx, int n, hipStream_t s);
@@ -135,9 +137,9 @@ hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView
                              const int* ib, const int* ie, MixedLineData out, hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s);
-// R = bins per lane (4 or 8); a workgroup covers mixed_bins_per_block(R) bins
+// R = bins per lane (4 or 8); a work item (one wave) covers mixed_bins_per_block(R) bins
 hipError_t launch_accum_mixed(const MixedPlan& plan, int R, hipStream_t s);
-int mixed_occupancy(int R);   // resident workgroups per CU of the mixed kernel
+int mixed_occupancy(int R);   // resident workgroups (kWavesPerGroup waves each) per CU of the mixed kernel
 // LDS bytes of a core-pass workgroup with the given row span (0: the row does not fit LDS)
 size_t core_lds_bytes(int span);
 int mixed_bins_per_block(int R);

@@ -324,35 +324,38 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
 // ---------------------------------------------------------------------------
 // Mixed-precision Voigt accumulation (persistent, work-queue driven).
 //
-// Work item = (point, bin block, line segment).  A workgroup of 4 waves owns
-// kBinsMixed = 4 x 64 x kR contiguous bins; lane l of wave w sums bins
-// w0 + l + 64q (q < kR), w0 = j0 + 64 kR w.  Its line segment is streamed
-// through LDS in 256-line tiles; per staged line (FP64 -> LDS):
+// Work item = (point, bin block, line segment); a block is the 64 R contiguous
+// bins of ONE wave (lane l sums bins w0 + l + 64q, q < R), and the four waves of
+// a workgroup take items independently: no workgroup barrier, so a wave whose
+// bins see fewer lines never waits for another.  A wave streams its line
+// segment in chunks of 64, one line per lane, staged FP64 -> FP32 into its own
+// LDS area:
 //   c      nearest bin to the centre (kept as an exact float integer)
 //   r      (nu - f_{c}) / alpha      (|r| <= e/2)
 //   e      dfreq / alpha
 // so x_j = r - (j - c) e with an exact integer factor and the FP32 offset
-// error stays relative to x itself.  Region I of W4 runs in FP32 (region1()),
-// and bins with |x| + y < 15 (the core, fixed per line by k_prep_mixed as an
-// integer range, so the region choice equals the FP64 path's) are left to the
-// core pass (k_core_mixed), which evaluates regions II-IV line by line.
-// Window and core tests are scalar per (wave, line): a wave whose bins all
-// lie inside the window and outside the core runs the branch-free loop.
-// Per-bin sums: plain FP32 over 32 lines, then a Kahan step into (sum, comp).
-// Segments of one block land in partial buffers and are added in segment
-// order by k_reduce_segments, so the result does not depend on scheduling.
+// error stays relative to x itself; lines with small e use anchored offsets
+// (one FMA per bin, below).  Region I of W4 runs in FP32 (region1()), and bins
+// with |x| + y < 15 (the core, fixed per line by k_prep_mixed as an integer
+// range, so the region choice equals the FP64 path's) are left to the core
+// pass (k_core_mixed), which evaluates regions II-IV line by line.
+// Window and core tests are scalar per (wave, line): a line whose window
+// covers all of the wave's bins and whose core misses them runs the
+// branch-free loop.  Per-bin sums: plain FP32 over 32 lines, then a Kahan
+// step into (sum, comp).  Segments of one block land in partial buffers and
+// are added in segment order by k_reduce_segments, so the result does not
+// depend on scheduling.
 // ---------------------------------------------------------------------------
 template <int R>
 struct MixedShape {
-  static constexpr int kWaveBins = 64 * R;
-  static constexpr int kBins = 4 * kWaveBins;
+  static constexpr int kBins = 64 * R;     // bins of one work item (one wave)
 };
 
 struct alignas(16) StagedLine {
   int ib, ie, k0, k1;         // window and core ranges (absolute bins)
   float e, h, k, a;           // e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
   float b, r, cf, anc;        // b = a (scaled); r = x at the centre bin; cf: centre bin relative
-                              // to the block origin; anc = 1: anchored offsets usable
+                              // to the wave's first bin; anc = 1: anchored offsets usable
 };
 
 // Anchored offsets: x at bin jb + o is A - o e, A = x(jb) rounded once from FP64 and o an
@@ -383,41 +386,53 @@ __device__ __forceinline__ float region1(float x, float h, float k, float a, flo
 
 __global__ void k_reset_counter(unsigned* c) { *c = 0u; }
 
-template <int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_accum_voigt_mixed(MixedPlan P) {
-  constexpr int kWaveBins = MixedShape<R>::kWaveBins;
-  constexpr int kBinsMixed = MixedShape<R>::kBins;
-  __shared__ StagedLine s_l[kTile];
-  __shared__ float4 s_anc[kTile * R];              // per line: x at the middle of each 64-bin slice
-  __shared__ int s_item;
+// LDS written by the lanes of one wave and read back by the same wave: LDS
+// instructions of a wave complete in order; this keeps the compiler from
+// moving accesses across the hand-over.
+__device__ __forceinline__ void wave_lds_handover() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
-  const int t = threadIdx.x;
-  const int lane = t & 63;
+template <int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ? 5 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
+  constexpr int kBins = MixedShape<R>::kBins;
+  constexpr int kWaves = kBlock / 64;
+  __shared__ StagedLine s_l[kWaves][64];
+  __shared__ float4 s_anc[kWaves][64][R / 4];     // per line: x at the middle of each 64-bin slice
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const float no = (float)(32 - lane);             // -(offset of this lane from its slice middle)
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const GridView gv = P.gv;
   const int jend = gv.start_idx + gv.points;
   const int per_task = P.nblocks * P.nseg;
+  StagedLine* my_l = s_l[wave];
+  float4* my_anc = &s_anc[wave][0][0];
+  // bin offsets of this lane from the wave's first bin, exact in FP32
+  float lanef_q[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) lanef_q[q] = (float)(lane + 64 * q);
 
+  int pass = 0;
   for (;;) {
-    if (t == 0) s_item = (int)atomicAdd(P.counter, 1u);
-    __syncthreads();
-    const int item = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
+    int item;
+    if (P.queue) {                                 // persistent: take items from the queue
+      unsigned v = 0;
+      if (lane == 0) v = atomicAdd(P.counter, 1u);
+      item = __builtin_amdgcn_readfirstlane((int)v);
+    } else {                                       // one item per wave
+      if (pass++) break;
+      item = blockIdx.x * kWaves + wave;
+    }
     if (item >= P.nitems) break;
     const int task = item / per_task;
     const int rem = item - task * per_task;
     const int b = rem / P.nseg;
     const int seg = rem - b * P.nseg;
     const MixedTask tk = P.tasks[task];
-    const int j0 = gv.start_idx + b * kBinsMixed;
-    const int w0 = j0 + wave * kWaveBins;
-    const int w1 = w0 + kWaveBins;
-    // bin offsets of this lane from the block origin, exact in FP32: x = (c - j) e + r
-    // is formed from the exact integer c - j, so its error stays relative to x itself
-    float lanef_q[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) lanef_q[q] = (float)(w0 - j0 + lane + 64 * q);
+    const int w0 = gv.start_idx + b * kBins;
+    const int w1 = w0 + kBins;
     const int2 rg = P.ranges[b];
     const int len = rg.y - rg.x;
     const int lo = rg.x + (int)(((long long)len * seg) / P.nseg);
@@ -427,9 +442,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 #pragma unroll
     for (int q = 0; q < R; ++q) sum[q] = comp[q] = part[q] = 0.0f;
     int since_flush = 0;
+    auto flush = [&]() {
+      if (++since_flush == 32) {
+        since_flush = 0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+      }
+    };
 
-    for (int base = lo; base < hi; base += kTile) {
-      const int i = base + t;
+    for (int base = lo; base < hi; base += 64) {
+      const int i = base + lane;
+      bool f = false, ed = false, an = false;
+      wave_lds_handover();                         // the previous chunk's reads are done
       if (i < hi) {
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
@@ -460,58 +484,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         L.k = (float)(2.0 * y2 * sg4);
         L.a = (float)(ac * sg2);
         L.b = (float)(ac * sg4);
-        L.cf = (float)(cj - (double)j0);
-        L.anc = gv.dfreq * g <= kAnchorMaxE ? 1.0f : 0.0f;
-        s_l[t] = L;
-        if (L.anc != 0.0f) {
-          // slice n = w R + q (wave w, slice q) has its middle at bin j0 + 64 n + 32
-          const double ed = gv.dfreq * g * sg;
-          const double x32 = (nu - fc) * g * sg + (cj - (double)(j0 + 32)) * ed;
-#pragma unroll 1
-          for (int n4 = 0; n4 < R; ++n4) {
-            const double b0 = x32 - (double)(256 * n4) * ed;
-            s_anc[t * R + n4] = make_float4((float)b0, (float)(b0 - 64.0 * ed), (float)(b0 - 128.0 * ed),
-                                            (float)(b0 - 192.0 * ed));
+        L.cf = (float)(cj - (double)w0);
+        an = gv.dfreq * g <= kAnchorMaxE;
+        L.anc = an ? 1.0f : 0.0f;
+        my_l[lane] = L;
+        const bool touch = L.ie > w0 && L.ib < w1;
+        const bool full = L.ib <= w0 && L.ie >= w1;
+        const bool cor = L.k1 > L.k0 && L.k1 > w0 && L.k0 < w1;
+        f = full && !cor;
+        ed = touch && !f;
+        if (f && an) {
+          // slice q of the wave has its middle at bin w0 + 64 q + 32
+          const double ed_ = gv.dfreq * g * sg;
+          const double x32 = (nu - fc) * g * sg + (cj - (double)(w0 + 32)) * ed_;
+#pragma unroll
+          for (int qg = 0; qg < R / 4; ++qg) {
+            const double b0 = x32 - (double)(256 * qg) * ed_;
+            my_anc[lane * (R / 4) + qg] = make_float4((float)b0, (float)(b0 - 64.0 * ed_),
+                                                      (float)(b0 - 128.0 * ed_), (float)(b0 - 192.0 * ed_));
           }
         }
       }
-      __syncthreads();
-      const int cnt = min(kTile, hi - base);
-      // Wave-level pruning: one ballot per 64 staged lines splits them into
-      //   fast  -- window covers all of this wave's bins, no W4 core among them
-      //   edge  -- touches the wave otherwise (partial window or core bins)
-      // and lines that miss the wave entirely are dropped.
-      unsigned long long fasta_m[kTile / 64], fastx_m[kTile / 64], edge_m[kTile / 64];
-#pragma unroll
-      for (int m = 0; m < kTile / 64; ++m) {
-        const int k = m * 64 + lane;
-        bool f = false, ed = false, an = false;
-        if (k < cnt) {
-          const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
-          const bool touch = I.y > w0 && I.x < w1;
-          const bool full = I.x <= w0 && I.y >= w1;
-          const bool core = I.w > I.z && I.w > w0 && I.z < w1;
-          f = full && !core;
-          ed = touch && !f;
-          an = s_l[k].anc != 0.0f;
-        }
-        fasta_m[m] = __ballot(f && an);
-        fastx_m[m] = __ballot(f && !an);
-        edge_m[m] = __ballot(ed);
-      }
-      auto flush = [&]() {
-        if (++since_flush == 32) {
-          since_flush = 0;
-#pragma unroll
-          for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
-        }
-      };
+      const unsigned long long fasta_m = __ballot(f && an);
+      const unsigned long long fastx_m = __ballot(f && !an);
+      unsigned long long edge_m = __ballot(ed);
+      wave_lds_handover();                         // staged records visible to every lane
+
       // Fast lines: branch-free FP32 region I for all R bins of every lane.  Two records
       // are in flight (ping-pong): the LDS read of one line overlaps the other's math.
-      auto sweep = [&](unsigned long long bits, int m, auto load, auto eval) {
+      auto sweep = [&](unsigned long long bits, auto load, auto eval) {
         if (!bits) return;
         auto next = [&]() {
-          const int k = m * 64 + __builtin_ctzll(bits);
+          const int k = __builtin_ctzll(bits);
           bits &= bits - 1;
           return k;
         };
@@ -540,10 +544,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
       };
       auto load_a = [&](int k) {
         RecA r;
-        r.E = *reinterpret_cast<const float4*>(&s_l[k].e);
-        r.b = s_l[k].b;
+        r.E = *reinterpret_cast<const float4*>(&my_l[k].e);
+        r.b = my_l[k].b;
 #pragma unroll
-        for (int qg = 0; qg < R / 4; ++qg) r.H[qg] = s_anc[k * R + wave * (R / 4) + qg];
+        for (int qg = 0; qg < R / 4; ++qg) r.H[qg] = my_anc[k * (R / 4) + qg];
         return r;
       };
       auto eval_a = [&](const RecA& r) {
@@ -558,8 +562,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
       };
       auto load_x = [&](int k) {
         RecX r;
-        r.E = *reinterpret_cast<const float4*>(&s_l[k].e);
-        r.B = *reinterpret_cast<const float4*>(&s_l[k].b);
+        r.E = *reinterpret_cast<const float4*>(&my_l[k].e);
+        r.B = *reinterpret_cast<const float4*>(&my_l[k].b);
         return r;
       };
       auto eval_x = [&](const RecX& r) {
@@ -568,31 +572,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
           part[q] += region1(__builtin_fmaf(r.B.z - lanef_q[q], r.E.x, r.B.y), r.E.y, r.E.z, r.E.w, r.B.x);
         flush();
       };
-#pragma unroll
-      for (int m = 0; m < kTile / 64; ++m) {
-        sweep(fasta_m[m], m, load_a, eval_a);
-        sweep(fastx_m[m], m, load_x, eval_x);
-      }
+      sweep(fasta_m, load_a, eval_a);
+      sweep(fastx_m, load_x, eval_x);
       // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
+      while (edge_m) {
+        const int k = __builtin_ctzll(edge_m);
+        edge_m &= edge_m - 1;
+        const int4 I = *reinterpret_cast<const int4*>(&my_l[k].ib);
+        const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
+        const float4 B = *reinterpret_cast<const float4*>(&my_l[k].b);
 #pragma unroll
-      for (int m = 0; m < kTile / 64; ++m) {
-        unsigned long long bits = edge_m[m];
-        while (bits) {
-          const int k = m * 64 + __builtin_ctzll(bits);
-          bits &= bits - 1;
-          const int4 I = *reinterpret_cast<const int4*>(&s_l[k].ib);
-          const float4 E = *reinterpret_cast<const float4*>(&s_l[k].e);
-          const float4 B = *reinterpret_cast<const float4*>(&s_l[k].b);
-#pragma unroll
-          for (int q = 0; q < R; ++q) {
-            const int jj = w0 + lane + 64 * q;
-            if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
-              part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
-          }
-          flush();
+        for (int q = 0; q < R; ++q) {
+          const int jj = w0 + lane + 64 * q;
+          if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
+            part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
         }
+        flush();
       }
-      __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
@@ -603,9 +599,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         if (jj < jend) tk.out[jj] += (double)sum[q] - (double)comp[q];
       }
     } else {
-      float2* dst = P.partial + (size_t)item * kBinsMixed;
+      float2* dst = P.partial + (size_t)item * kBins;
 #pragma unroll
-      for (int q = 0; q < R; ++q) dst[wave * kWaveBins + lane + 64 * q] = make_float2(sum[q], comp[q]);
+      for (int q = 0; q < R; ++q) dst[lane + 64 * q] = make_float2(sum[q], comp[q]);
     }
   }
 }

@@ -216,7 +216,7 @@ int launch_batch(gx_server* S, Slot& sl) {
   if ((rc = sl.rng.reserve(nblocks))) return rc;
   if (mixed) {
     if ((rc = sl.run.init())) return rc;
-    const int slots = sl.run.cus * mixed_occupancy(sl.run.R);
+    const int slots = sl.run.cus * mixed_occupancy(sl.run.R) * kWavesPerGroup;
     sl.run.forced_nseg = std::min(16, std::max(1, (4 * slots + nblocks - 1) / nblocks));
     sl.run.persistent = false;
     std::vector<MixedTask> tasks(B);
