@@ -43,7 +43,8 @@ class Consts(ctypes.Structure):
 
 class Timing(ctypes.Structure):
     _fields_ = [("intensity_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
-                ("evals", ctypes.c_longlong), ("lines", ctypes.c_longlong)]
+                ("evals", ctypes.c_longlong), ("lines", ctypes.c_longlong), ("region1_ms", ctypes.c_float),
+                ("core_ms", ctypes.c_float), ("core_evals", ctypes.c_longlong)]
 
 
 class Grid(ctypes.Structure):

@@ -387,21 +387,26 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
 }
 
 int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie,
-                        hipStream_t s) {
+                        hipStream_t s, hipEvent_t* ev) {
   if (plan.nitems <= 0) return GX_OK;
   plan.gv = gv;
   plan.nu = nu;
   plan.ranges = ranges;
   plan.ib = ib;
   plan.ie = ie;
-  hipError_t e = launch_accum_mixed(plan, R, s);
-  return e ? hip_fail(e, "mixed accumulate launch") : GX_OK;
+  hipError_t e;
+  if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");
+  if ((e = launch_accum_mixed(plan, R, s))) return hip_fail(e, "mixed accumulate launch");
+  if (ev && (e = hipEventRecord(ev[1], s))) return hip_fail(e, "event");
+  if ((e = launch_core_mixed(plan, s))) return hip_fail(e, "mixed core launch");
+  if (ev && (e = hipEventRecord(ev[2], s))) return hip_fail(e, "event");
+  return GX_OK;
 }
 
 // One (T,P) point over [start_idx, start_idx+points) of a grid: prep, ranges, accumulate.
 int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, int nlines, int precision,
               Derived& der, Ranges& rng, MixedRunner& mix, double* intens, unsigned long long* d_evals,
-              hipStream_t s, hipEvent_t* ev) {
+              unsigned long long* d_core_evals, hipStream_t s, hipEvent_t* ev) {
   LinesView lv{ls.nu, ls.elow, ls.aif, ls.gns, ls.gam, ls.nexp, nlines};
   int rc = der.reserve((size_t)std::max(nlines, 1));
   if (rc) return rc;
@@ -420,7 +425,9 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   if (mixed) {
     const WindowArgs wa{pc.cutoff, pc.convention, 1};
     if ((e = launch_windows(wa, gv, ls.nu, nlines, der.d.ib, der.d.ie, d_evals, s))) return hip_fail(e, "windows");
-    if ((e = launch_prep_mixed(der.d_pcs, 1, lv, gv, der.d.ib, der.d.ie, der.m, s)))
+    if (d_core_evals && (e = launch_fill(reinterpret_cast<double*>(d_core_evals), kEvalSlots, 0.0, s)))
+      return hip_fail(e, "fill");
+    if ((e = launch_prep_mixed(der.d_pcs, 1, lv, gv, der.d.ib, der.d.ie, der.m, d_core_evals, s)))
       return hip_fail(e, "mixed prep launch");
   } else {
     if ((e = launch_line_prep_f64(pc, gv, lv, der.d, d_evals, s))) return hip_fail(e, "line prep launch");
@@ -430,7 +437,7 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   if ((e = launch_block_ranges(gv, ls.nu, nlines, cut, bpb, nblocks, rng.r, s))) return hip_fail(e, "ranges launch");
   if (ev && (e = hipEventRecord(ev[2], s))) return hip_fail(e, "event");
   if (mixed) {
-    if ((rc = mix.launch(gv, ls.nu, rng.r, der.d.ib, der.d.ie, s))) return rc;
+    if ((rc = mix.launch(gv, ls.nu, rng.r, der.d.ib, der.d.ie, s, ev ? ev + 4 : nullptr))) return rc;
   } else {
     if ((e = launch_accum_f64(pc, gv, ls.nu, der.d, rng.r, nblocks, intens, s))) return hip_fail(e, "accumulate launch");
   }
@@ -448,8 +455,10 @@ struct gx_ctx {
   size_t budget = 0;
   size_t ntrans = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[7] = {};       // prep, ranges, accumulate; mixed: region I | core pass split
   bool timed = false;
+  bool last_mixed = false;
+  unsigned long long* d_core_evals = nullptr;   // kEvalSlots counters
   int M = 0;
   double* d_freq = nullptr;
   double* d_intens = nullptr;
@@ -498,7 +507,11 @@ int gx_create(int device, int profile, size_t mem_budget, gx_ctx** out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) { delete c; return hip_fail(e, "stream"); }
   for (auto& ev : c->ev)
     if ((e = hipEventCreate(&ev))) { gx_destroy(c); return hip_fail(e, "event"); }
-  if ((e = hipMalloc(&c->d_evals, sizeof(unsigned long long) * kEvalSlots))) { gx_destroy(c); return hip_fail(e, "hipMalloc"); }
+  if ((e = hipMalloc(&c->d_evals, sizeof(unsigned long long) * kEvalSlots)) ||
+      (e = hipMalloc(&c->d_core_evals, sizeof(unsigned long long) * kEvalSlots))) {
+    gx_destroy(c);
+    return hip_fail(e, "hipMalloc");
+  }
   *out = c;
   return GX_OK;
 }
@@ -588,8 +601,9 @@ int gx_execute(gx_ctx* c, int points, int n_ener, int start_idx) {
   if (e) return hip_fail(e, "execute: memset");
   c->last_lines = n_ener;
   c->timed = true;
-  return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->mix, c->d_intens, c->d_evals, c->stream,
-                   c->ev);
+  c->last_mixed = k.precision == GX_PREC_MIXED && c->profile == GX_VOIGT;
+  return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->mix, c->d_intens, c->d_evals,
+                   c->d_core_evals, c->stream, c->ev);
 }
 
 int gx_ready(gx_ctx* c) {
@@ -635,6 +649,7 @@ int gx_destroy(gx_ctx* c) {
   for (auto& ev : c->ev) if (ev) hipEventDestroy(ev);
   if (c->stream) hipStreamDestroy(c->stream);
   hipFree(c->d_evals);
+  hipFree(c->d_core_evals);
   delete c;
   return GX_OK;
 }
@@ -665,6 +680,20 @@ int gx_last_timing(gx_ctx* c, gx_timing* out) {
   out->total_ms = tot;
   out->evals = (long long)ev;
   out->lines = c->last_lines;
+  out->region1_ms = b;
+  out->core_ms = 0.0f;
+  out->core_evals = 0;
+  if (c->last_mixed) {
+    float r1 = 0, co = 0;
+    if ((e = hipEventElapsedTime(&r1, c->ev[4], c->ev[5])) || (e = hipEventElapsedTime(&co, c->ev[5], c->ev[6])))
+      return hip_fail(e, "last_timing: events");
+    unsigned long long cs[kEvalSlots], ce = 0;
+    if ((e = hipMemcpy(cs, c->d_core_evals, sizeof(cs), hipMemcpyDeviceToHost))) return hip_fail(e, "last_timing: copy");
+    for (unsigned long long v : cs) ce += v;
+    out->region1_ms = r1;
+    out->core_ms = co;
+    out->core_evals = (long long)ce;
+  }
   return GX_OK;
 }
 

@@ -93,16 +93,20 @@ struct MixedRunner {
   // chunk gv of the grid and nlines sorted lines, and fix the launch shape.
   // May synchronise the stream; never call it inside a graph capture.
   int prepare(const MixedTask* tasks, int ntasks, int nblocks, const GridView& gv, int nlines, hipStream_t s);
-  // Enqueue the kernels of the prepared plan (graph-capturable).
-  int launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie, hipStream_t s);
+  // Enqueue the kernels of the prepared plan (graph-capturable).  ev, when not NULL, gets
+  // 3 events: before region I, between region I and the core pass, after the core pass.
+  int launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie, hipStream_t s,
+             hipEvent_t* ev = nullptr);
   void release();
 };
 
 // Launch line prep, block ranges and accumulation of one point (asynchronous).
-// ev, when not NULL, holds 4 events recorded around the kernels.
+// ev, when not NULL, holds 7 events: 4 around prep / ranges / accumulation and,
+// for the mixed path, 3 around region I and the core pass.  d_core_evals (may be
+// NULL) counts the core-pass evaluations of the mixed path.
 int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, int nlines, int precision,
               Derived& der, Ranges& rng, MixedRunner& mix, double* intens, unsigned long long* d_evals,
-              hipStream_t s, hipEvent_t* ev);
+              unsigned long long* d_core_evals, hipStream_t s, hipEvent_t* ev);
 
 }  // namespace gx
 

@@ -133,12 +133,16 @@ hipError_t launch_accum_f64(const PointConsts& pc, const GridView& gv, const dou
                             const int2* ranges, int nblocks, double* intens, hipStream_t s);
 hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double* nu, int n, int* ib, int* ie,
                           unsigned long long* evals, hipStream_t s);
+// core_evals (kEvalSlots counters, may be NULL) accumulates the bins of all W4 cores
 hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, const GridView& gv,
-                             const int* ib, const int* ie, MixedLineData out, hipStream_t s);
+                             const int* ib, const int* ie, MixedLineData out, unsigned long long* core_evals,
+                             hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s);
-// R = bins per lane (4 or 8); a work item (one wave) covers mixed_bins_per_block(R) bins
+// R = bins per lane (4 or 8); a work item (one wave) covers mixed_bins_per_block(R) bins.
+// launch_accum_mixed: region I (and the segment reduction); launch_core_mixed: the W4 core pass.
 hipError_t launch_accum_mixed(const MixedPlan& plan, int R, hipStream_t s);
+hipError_t launch_core_mixed(const MixedPlan& plan, hipStream_t s);
 int mixed_occupancy(int R);   // resident workgroups (kWavesPerGroup waves each) per CU of the mixed kernel
 // LDS bytes of a core-pass workgroup with the given row span (0: the row does not fit LDS)
 size_t core_lds_bytes(int span);

@@ -194,14 +194,11 @@ __device__ __forceinline__ bool in_core(double nu, double g, double y, const dou
   return fabs(nu - freq[j]) * g + y < 15.0;
 }
 
-__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, LinesView lv, GridView gv,
-                                                    const int* __restrict__ IB, const int* __restrict__ IE,
-                                                    MixedLineData out) {
+__device__ __forceinline__ unsigned long long prep_mixed_line(const PointConsts& pc, const LinesView& lv,
+                                                              const GridView& gv, const int* __restrict__ IB,
+                                                              const int* __restrict__ IE, const MixedLineData& out,
+                                                              int i, int p) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = blockIdx.y;
-  if (i >= lv.n) return;
-  const PointConsts pc = pcs[p];
   const double nu = lv.nu[i];
   const double nu_if = nu == 0 ? 1e-6 : nu;
   const double a = lv.aif[i];
@@ -231,6 +228,16 @@ __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restric
   out.y[o] = y;
   out.g[o] = g;
   out.core[o] = make_int2(k0, k1);
+  return (unsigned long long)(k1 - k0);
+}
+
+__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, LinesView lv, GridView gv,
+                                                    const int* __restrict__ IB, const int* __restrict__ IE,
+                                                    MixedLineData out, unsigned long long* __restrict__ core_evals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long w = 0;
+  if (i < lv.n) w = prep_mixed_line(pcs[blockIdx.y], lv, gv, IB, IE, out, i, blockIdx.y);
+  if (core_evals) block_count(w, core_evals);
 }
 
 // Centre bin of a line on the grid (nearest node), clamped far inside int range.
@@ -841,9 +848,10 @@ hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double
 }
 
 hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView& lv, const GridView& gv,
-                             const int* ib, const int* ie, MixedLineData out, hipStream_t s) {
+                             const int* ib, const int* ie, MixedLineData out, unsigned long long* core_evals,
+                             hipStream_t s) {
   if (lv.n <= 0 || npts <= 0) return hipSuccess;
-  k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, gv, ib, ie, out);
+  k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, gv, ib, ie, out, core_evals);
   return hipGetLastError();
 }
 
@@ -884,8 +892,12 @@ hipError_t launch_accum_mixed(const MixedPlan& p, int R, hipStream_t s) {
     k_reduce_segments<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, bpb);
     if ((e = hipGetLastError())) return e;
   }
-  // W4 core pass
-  if (p.nlines <= 0 || p.ncwg <= 0 || p.gv.points <= 0) return hipSuccess;
+  return hipSuccess;
+}
+
+hipError_t launch_core_mixed(const MixedPlan& p, hipStream_t s) {
+  if (p.nitems <= 0 || p.nlines <= 0 || p.ncwg <= 0 || p.gv.points <= 0) return hipSuccess;
+  hipError_t e;
   k_centre_starts<<<(p.ncent + 1 + 255) / 256, 256, 0, s>>>(p.gv, p.nu, p.nlines, p.cbase, p.ncent, p.starts);
   if ((e = hipGetLastError())) return e;
   const size_t lds = core_lds_bytes(p.span);

@@ -234,7 +234,7 @@ int launch_batch(gx_server* S, Slot& sl) {
     const WindowArgs wa{pc0.cutoff, pc0.convention, 1};
     if (mixed) {
       if ((e = launch_windows(wa, gv, lv.nu, n, sl.ib, sl.ie, sl.evals, s))) return hip_fail(e, "server: windows");
-      if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, gv, sl.ib, sl.ie, sl.der.m, s))) return hip_fail(e, "server: prep");
+      if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, gv, sl.ib, sl.ie, sl.der.m, nullptr, s))) return hip_fail(e, "server: prep");
       if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
       if ((rc = sl.run.launch(gv, lv.nu, sl.rng.r, sl.ib, sl.ie, s))) return rc;
     } else {

@@ -53,6 +53,7 @@ struct Launch {               // one persistent accumulation launch (a batch of 
   int first = 0;              // index into the group's point list
   int count = 0;
   MixedRunner run;
+  hipEvent_t ev[3] = {};      // timed runs: region I | core pass
 };
 
 // autotune cache: (bins, lines, points per launch) -> (R, items per slot)
@@ -81,6 +82,8 @@ struct gx_table {
   hipGraphExec_t exec = nullptr;
   bool graph_ran = false;
   float eager_intensity_ms = 0, eager_accumulate_ms = 0;
+  float eager_region1_ms = 0, eager_core_ms = 0;
+  unsigned long long* core_evals = nullptr;   // kEvalSlots counters of the core pass
   int R = 4, ips = 4;
 };
 
@@ -106,6 +109,8 @@ int enqueue(gx_table* t, bool timed) {
   const LinesView lv = lines_view(t);
   if (timed && (e = hipEventRecord(t->ev[0], s))) return hip_fail(e, "event");
   if ((e = launch_fill(t->d_out, (size_t)t->M * t->pts.size(), 0.0, s))) return hip_fail(e, "fill");
+  if (t->core_evals && (e = launch_fill(reinterpret_cast<double*>(t->core_evals), kEvalSlots, 0.0, s)))
+    return hip_fail(e, "fill");
   for (Group& g : t->groups) {
     if ((e = launch_fill(reinterpret_cast<double*>(g.evals), kEvalSlots, 0.0, s))) return hip_fail(e, "fill");
     const WindowArgs wa{g.cutoff, t->o.convention, 1};
@@ -123,10 +128,11 @@ int enqueue(gx_table* t, bool timed) {
       const Group& g = t->groups[l.group];
       // points of one launch are contiguous in the constants table (grouped at create)
       const PointConsts* pc0 = t->der.d_pcs + g.first_const + l.first;
-      if ((e = launch_prep_mixed(pc0, l.count, lv, gv, g.ib, g.ie, t->der.m, s))) return hip_fail(e, "prep");
+      if ((e = launch_prep_mixed(pc0, l.count, lv, gv, g.ib, g.ie, t->der.m, t->core_evals, s)))
+        return hip_fail(e, "prep");
       if (first && timed && (e = hipEventRecord(t->ev[1], s))) return hip_fail(e, "event");
       first = false;
-      int rc = l.run.launch(gv, lv.nu, g.rng.r, g.ib, g.ie, s);
+      int rc = l.run.launch(gv, lv.nu, g.rng.r, g.ib, g.ie, s, timed && l.ev[0] ? l.ev : nullptr);
       if (rc) return rc;
     }
   } else {
@@ -221,6 +227,22 @@ int autotune(gx_table* t) {
   return prepare_launches(t);
 }
 
+// Region-I and core-pass times of the last timed run, summed over the launches.
+int split_times(gx_table* t, float* region1, float* core) {
+  *region1 = *core = 0.0f;
+  if (!mixed_table(t)) return GX_OK;
+  for (Launch& l : t->launches) {
+    if (!l.ev[0] || l.run.plan.nitems <= 0) continue;
+    float a = 0, b = 0;
+    hipError_t e;
+    if ((e = hipEventElapsedTime(&a, l.ev[0], l.ev[1])) || (e = hipEventElapsedTime(&b, l.ev[1], l.ev[2])))
+      return hip_fail(e, "table: events");
+    *region1 += a;
+    *core += b;
+  }
+  return GX_OK;
+}
+
 void free_table(gx_table* t) {
   if (!t) return;
   if (t->stream) (void)hipStreamSynchronize(t->stream);
@@ -233,7 +255,12 @@ void free_table(gx_table* t) {
     g.rng.release();
     g.rng64.release();
   }
-  for (Launch& l : t->launches) l.run.release();
+  for (Launch& l : t->launches) {
+    l.run.release();
+    for (auto& ev : l.ev)
+      if (ev) (void)hipEventDestroy(ev);
+  }
+  (void)hipFree(t->core_evals);
   t->der.release();
   t->der64.release();
   (void)hipFree(t->d_freq);
@@ -384,6 +411,11 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
         t->launches.back().count = std::min(t->batch, np - f0);
       }
     }
+    if ((e = hipMalloc(&t->core_evals, sizeof(unsigned long long) * kEvalSlots)))
+      return bail(hip_fail(e, "hipMalloc(core counters)"));
+    for (Launch& l : t->launches)
+      for (auto& ev : l.ev)
+        if ((e = hipEventCreate(&ev))) return bail(hip_fail(e, "event"));
     if ((rc = o->autotune ? autotune(t) : prepare_launches(t))) return bail(rc);
   } else {
     int rc = t->der64.reserve(std::max<size_t>(n, 1));
@@ -409,6 +441,7 @@ int gx_table_run(gx_table* t) {
     if ((e = hipEventElapsedTime(&t->eager_intensity_ms, t->ev[0], t->ev[1])) ||
         (e = hipEventElapsedTime(&t->eager_accumulate_ms, t->ev[1], t->ev[2])))
       return hip_fail(e, "table_run: events");
+    if ((rc = split_times(t, &t->eager_region1_ms, &t->eager_core_ms))) return rc;
     if ((e = hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal))) return hip_fail(e, "begin capture");
     rc = enqueue(t, false);
     hipGraph_t g = nullptr;
@@ -452,6 +485,25 @@ int gx_table_wait(gx_table* t, gx_timing* timing) {
   timing->total_ms = total;
   timing->evals = evals;
   timing->lines = (long long)t->L->ls.n;
+  timing->region1_ms = b;
+  timing->core_ms = 0.0f;
+  timing->core_evals = 0;
+  if (mixed_table(t)) {
+    if (t->graph_ran) {
+      timing->region1_ms = t->eager_region1_ms;
+      timing->core_ms = t->eager_core_ms;
+    } else {
+      int rc = split_times(t, &timing->region1_ms, &timing->core_ms);
+      if (rc) return rc;
+    }
+    if (t->core_evals) {
+      unsigned long long slots[kEvalSlots], ce = 0;
+      if ((e = hipMemcpy(slots, t->core_evals, sizeof(slots), hipMemcpyDeviceToHost)))
+        return hip_fail(e, "table_wait: copy");
+      for (auto v : slots) ce += v;
+      timing->core_evals = (long long)ce;
+    }
+  }
   return GX_OK;
 }
 

@@ -72,6 +72,10 @@ typedef struct gx_timing {
     float total_ms;         /* whole call */
     long long evals;        /* line-bin evaluations sum_i (ie_i - ib_i) of that call */
     long long lines;        /* lines processed */
+    float region1_ms;       /* mixed precision: the W4 region-I accumulation kernel alone
+                               (FP64: accumulate_ms) */
+    float core_ms;          /* mixed precision: the W4 core pass (regions II-IV); FP64: 0 */
+    long long core_evals;   /* mixed precision: evaluations made by the core pass; FP64: 0 */
 } gx_timing;
 
 /* BaseManager(ProfileType) + GpuManager ctor (GPUManager.cpp:5-17): bind

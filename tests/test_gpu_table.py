@@ -44,6 +44,10 @@ def test_mixed_table_vs_oracle_and_graph_replays(setup):
     eager.run()
     t = eager.wait()
     assert t.evals > 0
+    # the split of the accumulation into region I and the W4 core pass
+    assert 0 < t.core_evals < t.evals
+    assert t.region1_ms > 0 and t.core_ms > 0
+    assert t.region1_ms + t.core_ms <= t.accumulate_ms * 1.01 + 0.05
     ref_grids = [eager.read(p) for p in range(len(pts))]
     for p, (T, P, Q) in enumerate(pts):
         err = rel_err(ref_grids[p], oracle_grid(sub, T, P, Q, freq, d, 25.0))
