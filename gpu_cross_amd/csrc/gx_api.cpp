@@ -271,6 +271,26 @@ void MixedRunner::release() {
   h_tasks.clear();
 }
 
+double chunk_max_abs_nu(const GridView& gv) {
+  const double d = gv.dfreq;
+  return std::max(std::fabs(gv.start_nu + (double)(gv.start_idx - 1) * d),
+                  std::fabs(gv.start_nu + (double)(gv.start_idx + gv.points + 1) * d));
+}
+
+// A core bin j of a line satisfies |nu - f_j| < (15 - y) alpha <= 15 dpw nu, so
+// nu < F / (1 - 15 dpw) with F the largest |f| of the chunk, and the centre bin c is
+// within 15 dpw F / ((1 - 15 dpw) dnu) + 1/2 of j; kmax adds a margin of 3 bins.
+int core_kmax(const GridView& gv, double dpw) {
+  const double w = 15.0 * dpw;
+  if (!(w >= 0.0 && w < 0.5) || !(gv.dfreq > 0.0)) return -1;
+  const double kb = w * chunk_max_abs_nu(gv) / ((1.0 - w) * gv.dfreq);
+  if (!(kb < 1e6)) return -1;
+  const int kmax = (int)std::ceil(kb) + 3;
+  return core_lds_bytes(64 + 2 * kmax + 1) ? kmax : -1;
+}
+
+bool core_pass_fits(const GridView& gv, double dpw) { return gv.points <= 0 || core_kmax(gv, dpw) >= 0; }
+
 int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const GridView& gv, int nlines,
                          hipStream_t s) {
   int rc = init();
@@ -325,33 +345,23 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
   plan.grid = (int)((waves + kWavesPerGroup - 1) / kWavesPerGroup);
   plan.queue = persistent ? 1 : 0;
 
-  // Core pass geometry.  A core bin j of a line satisfies |nu - f_j| < (15 - y) alpha
-  // <= 15 dpw nu, so nu < F / (1 - 15 dpw) with F the largest |f| of the chunk, and the
-  // centre bin c is within kmax = 15 dpw F / ((1 - 15 dpw) dnu) + 1/2 (+ margin) of j.
   plan.nlines = std::max(nlines, 0);
   if (gv.points > 0 && plan.nlines > 0) {
     const double d = gv.dfreq;
-    const double F = std::max(std::fabs(gv.start_nu + (double)(gv.start_idx - 1) * d),
-                              std::fabs(gv.start_nu + (double)(gv.start_idx + gv.points + 1) * d));
-    double kb = 0.0;
+    int kmax = 0;
     for (int k = 0; k < ntasks; ++k) {
-      const double w = 15.0 * tasks[k].dpwcoef;
-      if (!(w >= 0.0 && w < 0.5)) return fail(GX_INVALID_ARG, "mixed: Doppler width coefficient out of range");
-      kb = std::max(kb, w * F / ((1.0 - w) * d));
+      const int kk = core_kmax(gv, tasks[k].dpwcoef);
+      if (kk < 0) return fail(GX_INVALID_ARG, "mixed: W4 core too wide for the core pass; use FP64 for this grid");
+      kmax = std::max(kmax, kk);
     }
-    if (!(kb < 1e6)) return fail(GX_INVALID_ARG, "mixed: W4 core wider than 1e6 bins; use FP64 for this grid");
-    const int kmax = (int)std::ceil(kb) + 3;
     // Centre bins per workgroup from the grid alone (a core half-width estimate at
     // dpw = 1e-5), so that a task's rows, hence its sums, do not depend on the other
     // tasks of the launch.  Row boundaries sit at multiples of cpw in absolute bins
     // for the same reason.
-    const double kg = std::min(1e6, 15.0 * 1e-5 * F / d);
+    const double kg = std::min(1e6, 15.0 * 1e-5 * chunk_max_abs_nu(gv) / d);
     int cpw = (int)std::min(256.0, std::max(64.0, std::ceil(2.0 * kg / 64.0) * 64.0));
     if (core_lds_bytes(cpw + 2 * kmax + 1) == 0) cpw = 64;
     const int span = cpw + 2 * kmax + 1;
-    if (core_lds_bytes(span) == 0)
-      return fail(GX_INVALID_ARG, "mixed: grid step too fine for the W4 core pass (core wider than 4000 bins); "
-                                  "use FP64 for this grid");
     const long long lo = (long long)gv.start_idx - kmax - 1;
     const long long cbase = (lo >= 0 ? lo / cpw : -((-lo + cpw - 1) / cpw)) * cpw;
     const long long cend = (long long)gv.start_idx + gv.points + kmax + 1;
@@ -410,7 +420,8 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   LinesView lv{ls.nu, ls.elow, ls.aif, ls.gns, ls.gam, ls.nexp, nlines};
   int rc = der.reserve((size_t)std::max(nlines, 1));
   if (rc) return rc;
-  const bool mixed = precision == GX_PREC_MIXED && pc.profile == GX_VOIGT;
+  // the mixed form where its W4 core pass fits the grid; the FP64 path (more accurate) otherwise
+  const bool mixed = precision == GX_PREC_MIXED && pc.profile == GX_VOIGT && core_pass_fits(gv, pc.dpwcoef);
   const int bpb = mixed ? mix.bins_per_block() : kBinsF64;
   const int nblocks = gv.points > 0 ? (gv.points + bpb - 1) / bpb : 0;
   if ((rc = rng.reserve(nblocks))) return rc;
@@ -601,7 +612,7 @@ int gx_execute(gx_ctx* c, int points, int n_ener, int start_idx) {
   if (e) return hip_fail(e, "execute: memset");
   c->last_lines = n_ener;
   c->timed = true;
-  c->last_mixed = k.precision == GX_PREC_MIXED && c->profile == GX_VOIGT;
+  c->last_mixed = k.precision == GX_PREC_MIXED && c->profile == GX_VOIGT && core_pass_fits(gv, pc.dpwcoef);
   return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->mix, c->d_intens, c->d_evals,
                    c->d_core_evals, c->stream, c->ev);
 }

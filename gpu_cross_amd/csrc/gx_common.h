@@ -100,6 +100,13 @@ struct MixedRunner {
   void release();
 };
 
+// Core pass of the mixed path: bound on |bin - centre bin| inside any W4 core of the
+// chunk gv at Doppler width coefficient dpw, or -1 when its bin row does not fit LDS
+// (such grids run the FP64 path).
+double chunk_max_abs_nu(const GridView& gv);
+int core_kmax(const GridView& gv, double dpw);
+bool core_pass_fits(const GridView& gv, double dpw);
+
 // Launch line prep, block ranges and accumulation of one point (asynchronous).
 // ev, when not NULL, holds 7 events: 4 around prep / ranges / accumulation and,
 // for the mixed path, 3 around region I and the core pass.  d_core_evals (may be

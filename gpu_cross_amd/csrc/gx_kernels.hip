@@ -369,8 +369,8 @@ struct alignas(16) StagedLine {
 // exact small integer, one FMA per bin.  With jb the middle of each 64-bin slice, |o| <= 32,
 // the error of x is eps (|x| + |A|) <= eps (2|x| + 32 e), and since region I has
 // |x| + y >= 15 the relative error of the profile stays below eps (1 + 3e); the form is
-// used for lines with e = dnu/alpha <= 2 (7 eps), the exact-integer form elsewhere.
-constexpr double kAnchorMaxE = 2.0;
+// used for lines with e = dnu/alpha <= 8 (25 eps), the exact-integer form elsewhere.
+constexpr double kAnchorMaxE = 8.0;
 
 __device__ __forceinline__ void kahan_step(float& sum, float& comp, float& part) {
   const float yk = part - comp;

@@ -453,6 +453,17 @@ int gx_server_submit(gx_server* S, const gx_request* r, uint64_t* id) {
   auto q = std::make_unique<Req>();
   q->r = *r;
   if (q->r.profile == GX_DOPPLER) q->r.precision = GX_PREC_FP64;
+  if (mixed_req(q->r)) {
+    // a grid too fine for the mixed path's W4 core pass runs on the FP64 path (decided from
+    // the request alone, so its result does not depend on what it is batched with)
+    gx_consts c{};
+    c.temperature = r->temperature;
+    c.mean_mass = S->o.mean_mass;
+    c.convention = S->o.convention;
+    const double d = (r->end_nu - r->start_nu) / double(r->npoints);
+    const GridView gv{nullptr, r->npoints, 0, r->npoints, r->start_nu, d};
+    if (!core_pass_fits(gv, make_point_consts(r->profile, c, r->pressure).dpwcoef)) q->r.precision = GX_PREC_FP64;
+  }
   q->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(S->mu);

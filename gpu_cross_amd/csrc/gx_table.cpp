@@ -380,6 +380,10 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
       grouped.push_back(t->pcs[p]);
     }
   }
+  // a grid too fine for the mixed path's W4 core pass at some point runs as an FP64 table
+  if (mixed_table(t))
+    for (int p = 0; p < npts; ++p)
+      if (!core_pass_fits(grid_view(t), t->pcs[p].dpwcoef)) t->o.precision = GX_PREC_FP64;
   for (Group& g : t->groups) {
     if ((e = hipMalloc(&g.ib, 4 * std::max<size_t>(n, 1))) || (e = hipMalloc(&g.ie, 4 * std::max<size_t>(n, 1))) ||
         (e = hipMalloc(&g.evals, sizeof(unsigned long long) * kEvalSlots)))

@@ -40,7 +40,8 @@ enum { GX_DOPPLER = 1, GX_VOIGT = 2 };
  * or the CUDA manager's (cross_kernal.cu:667,938,954-957, GPUManager.cpp:31) */
 enum { GX_CONV_OPENMP = 0, GX_CONV_GPU = 1 };
 /* FP64 throughout, or FP32 profiles with compensated per-bin sums and FP64
- * intensities / centre offsets */
+ * intensities / centre offsets (Voigt only; a grid so fine that a line's W4 core
+ * spans more than ~4000 bins runs the FP64 path instead) */
 enum { GX_PREC_FP64 = 0, GX_PREC_MIXED = 1 };
 
 typedef struct gx_ctx gx_ctx;

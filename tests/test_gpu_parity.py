@@ -142,6 +142,31 @@ def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
     assert err <= MIXED_TOL
 
 
+@pytest.mark.parametrize("width,kind", [(1.0, "core row in LDS, grid values from HBM"), (0.5, "FP64 fallback")])
+def test_voigt_mixed_fine_grids(voigt_lines, width, kind):
+    # 4000 bins over `width` cm^-1 at 10000 cm^-1, T = 3000 K: the W4 cores span ~2900 (5700) bins.
+    # At 1.0 cm^-1 the core pass keeps its bin row in LDS but reads the grid from HBM; at 0.5 cm^-1
+    # the row no longer fits and the point runs on the FP64 path, held to the FP64 bar.
+    ll = voigt_lines
+    T, P = 3000.0, 1e-3
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 9970.0) & (ll.nu < 10030.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(10000.0, 10000.0 + width, 4000)
+    gam = L.gamma_at(P)
+    print(f"voigt mixed fine grid width={width} ({kind}) lines={L.n}", flush=True)
+    out, t = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                         cutoff=25.0)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=25.0)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(out, ref)
+    print(f"  max rel err {err:.3e}, core evals {t.core_evals}", flush=True)
+    if kind == "FP64 fallback":
+        assert err <= FP64_TOL and t.core_evals == 0
+    else:
+        assert err <= MIXED_TOL and t.core_evals > 0
+
+
 def test_fp64_deterministic_and_linear(voigt_lines):
     ll = voigt_lines
     T, P = 900.0, 1.0
