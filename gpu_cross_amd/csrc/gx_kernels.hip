@@ -412,6 +412,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const float no = (float)(32 - lane);             // -(offset of this lane from its slice middle)
   const GridView gv = P.gv;
+  const double inv_dfreq = 1.0 / gv.dfreq;
   const int jend = gv.start_idx + gv.points;
   const int per_task = P.nblocks * P.nseg;
   StagedLine* my_l = s_l[wave];
@@ -449,26 +450,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
 #pragma unroll
     for (int q = 0; q < R; ++q) sum[q] = comp[q] = part[q] = 0.0f;
     int since_flush = 0;
-    auto flush = [&]() {
-      if (++since_flush == 32) {
-        since_flush = 0;
+
+    // Kahan step after every 32 lines: the lines of a class are added in blocks that
+    // end where the 32-line count does, so the flush test runs once per block
+    auto run_block = [&](int n, auto body) {
+      for (int i = 0; i < n;) {
+        const int m = min(n - i, 32 - since_flush);
+        body(i, i + m);
+        i += m;
+        since_flush += m;
+        if (since_flush == 32) {
+          since_flush = 0;
 #pragma unroll
-        for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+          for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+        }
       }
     };
 
     for (int base = lo; base < hi; base += 64) {
       const int i = base + lane;
-      bool f = false, ed = false, an = false;
-      wave_lds_handover();                         // the previous chunk's reads are done
+      bool fa = false, fx = false, ed = false;
+      StagedLine L;
+      float4 anc4[R / 4];
       if (i < hi) {
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
         const double nu = v0 == 0.0 ? 1e-6 : v0;
         const double g = tk.g[i], y = tk.y[i];                      // k_prep_mixed: FP64 path's bits
-        const double cj = rint((nu - gv.start_nu) / gv.dfreq);      // centre bin
+        const double cj = rint((nu - gv.start_nu) * inv_dfreq);     // nearest bin (r below is consistent)
         const double fc = gv.start_nu + cj * gv.dfreq;
-        StagedLine L;
         L.ib = P.ib[i];
         L.ie = P.ie[i];
         const int2 core = tk.core[i];
@@ -481,9 +491,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
         const double xmax = (fabs((double)L.ib - cj) + fabs((double)L.ie - cj) + 2.0) * gv.dfreq * g;
         const int kexp = xmax > 65536.0 ? (int)ceil(log2(xmax / 65536.0)) : 0;
         const double sg = ldexp(1.0, -kexp), sg2 = sg * sg, sg4 = sg2 * sg2;
-        const float e = (float)(gv.dfreq * g * sg);
         L.r = (float)((nu - fc) * g * sg);
-        L.e = e;
+        L.e = (float)(gv.dfreq * g * sg);
         // scaled: u' = x'^2 + (y^2 - 1/2) s^2, den' = u'^2 + 2 y^2 s^4, num' = a s^2 (u' + s^2)
         const double y2 = y * y;
         const double ac = (double)amp * 0.5641896 * y;
@@ -492,110 +501,81 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
         L.a = (float)(ac * sg2);
         L.b = (float)(ac * sg4);
         L.cf = (float)(cj - (double)w0);
-        an = gv.dfreq * g <= kAnchorMaxE;
+        const bool an = gv.dfreq * g <= kAnchorMaxE;
         L.anc = an ? 1.0f : 0.0f;
-        my_l[lane] = L;
         const bool touch = L.ie > w0 && L.ib < w1;
         const bool full = L.ib <= w0 && L.ie >= w1;
         const bool cor = L.k1 > L.k0 && L.k1 > w0 && L.k0 < w1;
-        f = full && !cor;
-        ed = touch && !f;
-        if (f && an) {
+        fa = full && !cor && an;
+        fx = full && !cor && !an;
+        ed = touch && !(full && !cor);
+        if (fa) {
           // slice q of the wave has its middle at bin w0 + 64 q + 32
           const double ed_ = gv.dfreq * g * sg;
           const double x32 = (nu - fc) * g * sg + (cj - (double)(w0 + 32)) * ed_;
 #pragma unroll
           for (int qg = 0; qg < R / 4; ++qg) {
             const double b0 = x32 - (double)(256 * qg) * ed_;
-            my_anc[lane * (R / 4) + qg] = make_float4((float)b0, (float)(b0 - 64.0 * ed_),
-                                                      (float)(b0 - 128.0 * ed_), (float)(b0 - 192.0 * ed_));
+            anc4[qg] = make_float4((float)b0, (float)(b0 - 64.0 * ed_), (float)(b0 - 128.0 * ed_),
+                                   (float)(b0 - 192.0 * ed_));
           }
         }
       }
-      const unsigned long long fasta_m = __ballot(f && an);
-      const unsigned long long fastx_m = __ballot(f && !an);
-      unsigned long long edge_m = __ballot(ed);
+      // Compact the chunk's records by class, in line order: anchored fast lines first, then
+      // exact-offset fast lines, then edge lines; lines that miss the wave are dropped.
+      const unsigned long long ma = __ballot(fa), mx = __ballot(fx), me = __ballot(ed);
+      const int na = __popcll(ma), nx = __popcll(mx), ne = __popcll(me);
+      const unsigned long long mine = fa ? ma : (fx ? mx : me);
+      const int pos = (fx ? na : 0) + (ed ? na + nx : 0) + __builtin_amdgcn_mbcnt_hi(
+          (unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+      wave_lds_handover();                         // the previous chunk's reads are done
+      if (fa || fx || ed) {
+        my_l[pos] = L;
+        if (fa) {
+#pragma unroll
+          for (int qg = 0; qg < R / 4; ++qg) my_anc[pos * (R / 4) + qg] = anc4[qg];
+        }
+      }
       wave_lds_handover();                         // staged records visible to every lane
 
-      // Fast lines: branch-free FP32 region I for all R bins of every lane.  Two records
-      // are in flight (ping-pong): the LDS read of one line overlaps the other's math.
-      auto sweep = [&](unsigned long long bits, auto load, auto eval) {
-        if (!bits) return;
-        auto next = [&]() {
-          const int k = __builtin_ctzll(bits);
-          bits &= bits - 1;
-          return k;
-        };
-        auto A = load(next());
-        decltype(A) B;
-        bool hb = bits != 0;
-        if (hb) B = load(next());
-        for (;;) {
-          eval(A);
-          if (!hb) break;
-          const bool ha = bits != 0;
-          if (ha) A = load(next());
-          eval(B);
-          if (!ha) break;
-          hb = bits != 0;
-          if (hb) B = load(next());
+      // Fast lines: branch-free FP32 region I for all R bins of every lane.
+      run_block(na, [&](int a, int z) {
+        for (int k = a; k < z; ++k) {
+          const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
+          const float bb = my_l[k].b;
+#pragma unroll
+          for (int qg = 0; qg < R / 4; ++qg) {
+            const float4 H = my_anc[k * (R / 4) + qg];
+            const float hq[4] = {H.x, H.y, H.z, H.w};
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
+              part[4 * qg + qq] += region1(__builtin_fmaf(no, E.x, hq[qq]), E.y, E.z, E.w, bb);
+          }
         }
-      };
-      struct RecA {
-        float4 E;                 // e, h, k, a
-        float b;
-        float4 H[R / 4];          // x at this wave's slice middles
-      };
-      struct RecX {
-        float4 E, B;              // (e, h, k, a), (b, r, cf, -)
-      };
-      auto load_a = [&](int k) {
-        RecA r;
-        r.E = *reinterpret_cast<const float4*>(&my_l[k].e);
-        r.b = my_l[k].b;
+      });
+      run_block(nx, [&](int a, int z) {
+        for (int k = na + a; k < na + z; ++k) {
+          const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
+          const float4 B = *reinterpret_cast<const float4*>(&my_l[k].b);
 #pragma unroll
-        for (int qg = 0; qg < R / 4; ++qg) r.H[qg] = my_anc[k * (R / 4) + qg];
-        return r;
-      };
-      auto eval_a = [&](const RecA& r) {
-#pragma unroll
-        for (int qg = 0; qg < R / 4; ++qg) {
-          const float hq[4] = {r.H[qg].x, r.H[qg].y, r.H[qg].z, r.H[qg].w};
-#pragma unroll
-          for (int qq = 0; qq < 4; ++qq)
-            part[4 * qg + qq] += region1(__builtin_fmaf(no, r.E.x, hq[qq]), r.E.y, r.E.z, r.E.w, r.b);
-        }
-        flush();
-      };
-      auto load_x = [&](int k) {
-        RecX r;
-        r.E = *reinterpret_cast<const float4*>(&my_l[k].e);
-        r.B = *reinterpret_cast<const float4*>(&my_l[k].b);
-        return r;
-      };
-      auto eval_x = [&](const RecX& r) {
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          part[q] += region1(__builtin_fmaf(r.B.z - lanef_q[q], r.E.x, r.B.y), r.E.y, r.E.z, r.E.w, r.B.x);
-        flush();
-      };
-      sweep(fasta_m, load_a, eval_a);
-      sweep(fastx_m, load_x, eval_x);
-      // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
-      while (edge_m) {
-        const int k = __builtin_ctzll(edge_m);
-        edge_m &= edge_m - 1;
-        const int4 I = *reinterpret_cast<const int4*>(&my_l[k].ib);
-        const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
-        const float4 B = *reinterpret_cast<const float4*>(&my_l[k].b);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          const int jj = w0 + lane + 64 * q;
-          if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
+          for (int q = 0; q < R; ++q)
             part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
         }
-        flush();
-      }
+      });
+      // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
+      run_block(ne, [&](int a, int z) {
+        for (int k = na + nx + a; k < na + nx + z; ++k) {
+          const int4 I = *reinterpret_cast<const int4*>(&my_l[k].ib);
+          const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
+          const float4 B = *reinterpret_cast<const float4*>(&my_l[k].b);
+#pragma unroll
+          for (int q = 0; q < R; ++q) {
+            const int jj = w0 + lane + 64 * q;
+            if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
+              part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
+          }
+        }
+      });
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
