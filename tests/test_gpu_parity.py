@@ -221,3 +221,27 @@ def test_voigt_mixed_tiny_nu_lines():
     err = rel_err(out, ref)
     print(f"  tiny-nu mixed max rel err {err:.3e}", flush=True)
     assert err <= MIXED_TOL
+
+
+@pytest.mark.parametrize("R", ["4", "8"])
+def test_voigt_mixed_bins_per_lane_and_repeatability(voigt_lines, monkeypatch, R):
+    """Both launch shapes of the mixed kernel (4 and 8 bins per lane) against the oracle, and
+    the same inputs twice give the same grid bit for bit (fixed summation order)."""
+    monkeypatch.setenv("GX_MIXED_R", R)
+    ll = voigt_lines
+    T, P = 800.0, 0.01
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 4450.0) & (ll.nu < 4750.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(4500.0, 4700.0, 40000)
+    gam = L.gamma_at(P)
+    print(f"voigt mixed R={R} lines={L.n}", flush=True)
+    a, t = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED, cutoff=25.0)
+    b, _ = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED, cutoff=25.0)
+    assert np.array_equal(a, b)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=25.0)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(a, ref)
+    print(f"  max rel err {err:.3e}, core evals {t.core_evals} of {t.evals}", flush=True)
+    assert err <= MIXED_TOL
+    assert 0 < t.core_evals < t.evals
