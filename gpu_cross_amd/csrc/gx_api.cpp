@@ -422,6 +422,7 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   if (rc) return rc;
   // the mixed form where its W4 core pass fits the grid; the FP64 path (more accurate) otherwise
   const bool mixed = precision == GX_PREC_MIXED && pc.profile == GX_VOIGT && core_pass_fits(gv, pc.dpwcoef);
+  if (mixed && (rc = mix.init())) return rc;   // fixes R (GX_MIXED_R) before the block size is used
   const int bpb = mixed ? mix.bins_per_block() : kBinsF64;
   const int nblocks = gv.points > 0 ? (gv.points + bpb - 1) / bpb : 0;
   if ((rc = rng.reserve(nblocks))) return rc;

@@ -211,6 +211,7 @@ int launch_batch(gx_server* S, Slot& sl) {
     if ((rc = sl.der.reserve((size_t)std::max(n, 1)))) return rc;
   }
   // launch shape from the request's own grid only (alone == in a mix)
+  if (mixed && (rc = sl.run.init())) return rc;   // fixes R (GX_MIXED_R) before the block size is used
   const int bpb = mixed ? sl.run.bins_per_block() : kBinsF64;
   const int nblocks = (M + bpb - 1) / bpb;
   if ((rc = sl.rng.reserve(nblocks))) return rc;
