@@ -17,10 +17,12 @@
 //   double start_nu, end_nu, temperature, partition, mean_mass, pressure, ref_temp
 //   int64  nlines
 //   double energies[n], nu[n], aif[n]; int32 gns[n]; double gamma[n], nexp[n]  (last two Voigt only)
-// Output file: double intens[npoints]
+// Output file: double intens[npoints].  stderr: "manager_seconds <s>", the wall time from the
+// first TransferVectors to TransferResults (the CPU baseline timing; file I/O excluded).
 #include <malloc.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -81,6 +83,7 @@ int main(int argc, char** argv) {
   std::vector<int> hg(fit);
   int ntrans = 0;
   double min_nu = 99999.0, max_nu = 0.0;
+  const auto t_start = std::chrono::steady_clock::now();
   auto perform = [&]() {  // BaseProfile.cpp:49-62
     mgr->TransferVectors(ntrans, hE.data(), hnu.data(), hA.data(), hg.data(),
                          profile == VOIGT ? hgam.data() : nullptr, profile == VOIGT ? hn.data() : nullptr);
@@ -110,6 +113,8 @@ int main(int argc, char** argv) {
   std::vector<double> tmp_freq(npoints), tmp(npoints, 0.0);
   mgr->TransferResults(tmp_freq.data(), tmp.data(), npoints);
   for (int i = 0; i < npoints; i++) intens[i] += tmp[i];
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  fprintf(stderr, "manager_seconds %.6f\n", secs);
 
   FILE* out = fopen(argv[2], "wb");
   if (!out) { perror("output"); return 2; }
