@@ -348,8 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
 // pass (k_core_mixed), which evaluates regions II-IV line by line.
 // Window and core tests are scalar per (wave, line): a line whose window
 // covers all of the wave's bins and whose core misses them runs the
-// branch-free loop.  Per-bin sums: plain FP32 over 32 lines, then a Kahan
-// step into (sum, comp).  Segments of one block land in partial buffers and
+// branch-free loop.  Per-bin sums: plain FP32 over 32 lines, then a compensated
+// step into (sum, part) (kahan_step).  Segments of one block land in partial buffers and
 // are added in segment order by k_reduce_segments, so the result does not
 // depend on scheduling.
 // ---------------------------------------------------------------------------
@@ -358,11 +358,17 @@ struct MixedShape {
   static constexpr int kBins = 64 * R;     // bins of one work item (one wave)
 };
 
-struct alignas(16) StagedLine {
-  int ib, ie, k0, k1;         // window and core ranges (absolute bins)
-  float e, h, k, a;           // e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
-  float b, r, cf, anc;        // b = a (scaled); r = x at the centre bin; cf: centre bin relative
-                              // to the wave's first bin; anc = 1: anchored offsets usable
+// Staged line record, 2 + R/4 float4 per line:
+//   [0]      ib, ie, k0, k1 (ints: window and core ranges, absolute bins); anchored fast
+//            lines keep x at the middles of slices 0-3 here instead (they need no ranges)
+//   [1]      e, h, k, a      e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
+//   [2]      b, r, cf, -     b = a (scaled); r = x at the centre bin; cf: centre bin
+//                            relative to the wave's first bin
+//   [3 ...]  anchored fast lines: x at the middles of slices 4-7 (R = 8)
+// so the fast loop reads everything of a line from one LDS address.
+template <int R>
+struct StagedRec {
+  static constexpr int kVec = 2 + R / 4;
 };
 
 // Anchored offsets: x at bin jb + o is A - o e, A = x(jb) rounded once from FP64 and o an
@@ -372,12 +378,14 @@ struct alignas(16) StagedLine {
 // used for lines with e = dnu/alpha <= 8 (25 eps), the exact-integer form elsewhere.
 constexpr double kAnchorMaxE = 8.0;
 
-__device__ __forceinline__ void kahan_step(float& sum, float& comp, float& part) {
-  const float yk = part - comp;
-  const float tk = sum + yk;
-  comp = (tk - sum) - yk;
+// Kahan summation with the compensation kept in the running partial sum: part holds
+// the terms added since the last step plus the rounding error of that step (Fast2Sum
+// error, exact while |sum| >= |part|, as in Kahan's form; all terms here are >= 0), so
+// the next partial sum starts from the correction and the value is sum + part.
+__device__ __forceinline__ void kahan_step(float& sum, float& part) {
+  const float tk = sum + part;
+  part = part - (tk - sum);
   sum = tk;
-  part = 0.0f;
 }
 
 // W4 region I, c y (1/2 + x^2 + y^2) / ((1/2 + y^2 - x^2)^2 + 4 x^2 y^2), rewritten with
@@ -405,8 +413,8 @@ template <int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ? 5 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
   constexpr int kBins = MixedShape<R>::kBins;
   constexpr int kWaves = kBlock / 64;
-  __shared__ StagedLine s_l[kWaves][64];
-  __shared__ float4 s_anc[kWaves][64][R / 4];     // per line: x at the middle of each 64-bin slice
+  constexpr int kVec = StagedRec<R>::kVec;
+  __shared__ float4 s_l[kWaves][64][kVec];
 
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -415,8 +423,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
   const double inv_dfreq = 1.0 / gv.dfreq;
   const int jend = gv.start_idx + gv.points;
   const int per_task = P.nblocks * P.nseg;
-  StagedLine* my_l = s_l[wave];
-  float4* my_anc = &s_anc[wave][0][0];
+  float4 (*my_l)[kVec] = s_l[wave];
   // bin offsets of this lane from the wave's first bin, exact in FP32
   float lanef_q[R];
 #pragma unroll
@@ -446,9 +453,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
     const int lo = rg.x + (int)(((long long)len * seg) / P.nseg);
     const int hi = rg.x + (int)(((long long)len * (seg + 1)) / P.nseg);
 
-    float sum[R], comp[R], part[R];
+    float sum[R], part[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) sum[q] = comp[q] = part[q] = 0.0f;
+    for (int q = 0; q < R; ++q) sum[q] = part[q] = 0.0f;
     int since_flush = 0;
 
     // Kahan step after every 32 lines: the lines of a class are added in blocks that
@@ -462,7 +469,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
         if (since_flush == 32) {
           since_flush = 0;
 #pragma unroll
-          for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+          for (int q = 0; q < R; ++q) kahan_step(sum[q], part[q]);
         }
       }
     };
@@ -470,8 +477,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
     for (int base = lo; base < hi; base += 64) {
       const int i = base + lane;
       bool fa = false, fx = false, ed = false;
-      StagedLine L;
-      float4 anc4[R / 4];
+      float4 rec[kVec];
       if (i < hi) {
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
@@ -479,33 +485,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
         const double g = tk.g[i], y = tk.y[i];                      // k_prep_mixed: FP64 path's bits
         const double cj = rint((nu - gv.start_nu) * inv_dfreq);     // nearest bin (r below is consistent)
         const double fc = gv.start_nu + cj * gv.dfreq;
-        L.ib = P.ib[i];
-        L.ie = P.ie[i];
+        const int ib = P.ib[i], ie = P.ie[i];
         const int2 core = tk.core[i];
-        L.k0 = core.x;
-        L.k1 = core.y;
         const float amp = tk.amp[i];
         // Power-of-two rescaling x' = sigma x keeps x'^4 far from FP32 overflow for lines
         // whose window spans huge x (tiny nu => huge 1/alpha); region I is invariant under
         // (A1, A2, R, P2) -> (A1 s^4, A2 s^2, R s^2, P2 s^4).  sigma = 1 for ordinary lines.
-        const double xmax = (fabs((double)L.ib - cj) + fabs((double)L.ie - cj) + 2.0) * gv.dfreq * g;
+        const double xmax = (fabs((double)ib - cj) + fabs((double)ie - cj) + 2.0) * gv.dfreq * g;
         const int kexp = xmax > 65536.0 ? (int)ceil(log2(xmax / 65536.0)) : 0;
         const double sg = ldexp(1.0, -kexp), sg2 = sg * sg, sg4 = sg2 * sg2;
-        L.r = (float)((nu - fc) * g * sg);
-        L.e = (float)(gv.dfreq * g * sg);
         // scaled: u' = x'^2 + (y^2 - 1/2) s^2, den' = u'^2 + 2 y^2 s^4, num' = a s^2 (u' + s^2)
         const double y2 = y * y;
         const double ac = (double)amp * 0.5641896 * y;
-        L.h = (float)((y2 - 0.5) * sg2);
-        L.k = (float)(2.0 * y2 * sg4);
-        L.a = (float)(ac * sg2);
-        L.b = (float)(ac * sg4);
-        L.cf = (float)(cj - (double)w0);
+        rec[1] = make_float4((float)(gv.dfreq * g * sg), (float)((y2 - 0.5) * sg2),
+                             (float)(2.0 * y2 * sg4), (float)(ac * sg2));
+        rec[2] = make_float4((float)(ac * sg4), (float)((nu - fc) * g * sg),
+                             (float)(cj - (double)w0), 0.0f);
         const bool an = gv.dfreq * g <= kAnchorMaxE;
-        L.anc = an ? 1.0f : 0.0f;
-        const bool touch = L.ie > w0 && L.ib < w1;
-        const bool full = L.ib <= w0 && L.ie >= w1;
-        const bool cor = L.k1 > L.k0 && L.k1 > w0 && L.k0 < w1;
+        const bool touch = ie > w0 && ib < w1;
+        const bool full = ib <= w0 && ie >= w1;
+        const bool cor = core.y > core.x && core.y > w0 && core.x < w1;
         fa = full && !cor && an;
         fx = full && !cor && !an;
         ed = touch && !(full && !cor);
@@ -516,9 +515,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
 #pragma unroll
           for (int qg = 0; qg < R / 4; ++qg) {
             const double b0 = x32 - (double)(256 * qg) * ed_;
-            anc4[qg] = make_float4((float)b0, (float)(b0 - 64.0 * ed_), (float)(b0 - 128.0 * ed_),
-                                   (float)(b0 - 192.0 * ed_));
+            rec[qg == 0 ? 0 : 2 + qg] = make_float4((float)b0, (float)(b0 - 64.0 * ed_),
+                                                    (float)(b0 - 128.0 * ed_), (float)(b0 - 192.0 * ed_));
           }
+        } else {
+          rec[0] = make_float4(__int_as_float(ib), __int_as_float(ie), __int_as_float(core.x),
+                               __int_as_float(core.y));
         }
       }
       // Compact the chunk's records by class, in line order: anchored fast lines first, then
@@ -530,22 +532,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
           (unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
       wave_lds_handover();                         // the previous chunk's reads are done
       if (fa || fx || ed) {
-        my_l[pos] = L;
+        my_l[pos][0] = rec[0];
+        my_l[pos][1] = rec[1];
+        my_l[pos][2] = rec[2];
         if (fa) {
 #pragma unroll
-          for (int qg = 0; qg < R / 4; ++qg) my_anc[pos * (R / 4) + qg] = anc4[qg];
+          for (int v = 3; v < kVec; ++v) my_l[pos][v] = rec[v];
         }
       }
       wave_lds_handover();                         // staged records visible to every lane
 
       // Fast lines: branch-free FP32 region I for all R bins of every lane.
       run_block(na, [&](int a, int z) {
+#pragma unroll 2
         for (int k = a; k < z; ++k) {
-          const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
-          const float bb = my_l[k].b;
+          const float4 E = my_l[k][1];
+          const float bb = my_l[k][2].x;
 #pragma unroll
           for (int qg = 0; qg < R / 4; ++qg) {
-            const float4 H = my_anc[k * (R / 4) + qg];
+            const float4 H = my_l[k][qg == 0 ? 0 : 2 + qg];
             const float hq[4] = {H.x, H.y, H.z, H.w};
 #pragma unroll
             for (int qq = 0; qq < 4; ++qq)
@@ -555,8 +560,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
       });
       run_block(nx, [&](int a, int z) {
         for (int k = na + a; k < na + z; ++k) {
-          const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
-          const float4 B = *reinterpret_cast<const float4*>(&my_l[k].b);
+          const float4 E = my_l[k][1];
+          const float4 B = my_l[k][2];
 #pragma unroll
           for (int q = 0; q < R; ++q)
             part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
@@ -565,9 +570,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
       // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
       run_block(ne, [&](int a, int z) {
         for (int k = na + nx + a; k < na + nx + z; ++k) {
-          const int4 I = *reinterpret_cast<const int4*>(&my_l[k].ib);
-          const float4 E = *reinterpret_cast<const float4*>(&my_l[k].e);
-          const float4 B = *reinterpret_cast<const float4*>(&my_l[k].b);
+          const float4 Iv = my_l[k][0];
+          const int4 I = make_int4(__float_as_int(Iv.x), __float_as_int(Iv.y), __float_as_int(Iv.z),
+                                   __float_as_int(Iv.w));
+          const float4 E = my_l[k][1];
+          const float4 B = my_l[k][2];
 #pragma unroll
           for (int q = 0; q < R; ++q) {
             const int jj = w0 + lane + 64 * q;
@@ -578,22 +585,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
       });
     }
 #pragma unroll
-    for (int q = 0; q < R; ++q) kahan_step(sum[q], comp[q], part[q]);
+    for (int q = 0; q < R; ++q) kahan_step(sum[q], part[q]);
     if (P.nseg == 1) {
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int jj = w0 + lane + 64 * q;
-        if (jj < jend) tk.out[jj] += (double)sum[q] - (double)comp[q];
+        if (jj < jend) tk.out[jj] += (double)sum[q] + (double)part[q];
       }
     } else {
       float2* dst = P.partial + (size_t)item * kBins;
 #pragma unroll
-      for (int q = 0; q < R; ++q) dst[lane + 64 * q] = make_float2(sum[q], comp[q]);
+      for (int q = 0; q < R; ++q) dst[lane + 64 * q] = make_float2(sum[q], -part[q]);
     }
   }
 }
 
-// out[j] += sum over segments, in segment order, of (sum - comp)
+// out[j] += sum over segments, in segment order, of (sum - comp); the region-I kernel
+// writes comp = -part
 __global__ void k_reduce_segments(MixedPlan P, int kBinsMixed) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long per_task = (long long)P.nblocks * kBinsMixed;
