@@ -142,6 +142,28 @@ def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
     assert err <= MIXED_TOL
 
 
+@pytest.mark.parametrize("T,cutoff", [(300.0, 0.3), (300.0, 1.2), (3000.0, 1.2), (3000.0, 0.3), (1000.0, 3.0)])
+def test_voigt_mixed_short_windows(voigt_lines, T, cutoff):
+    # Windows of 120-1200 bins against 256-bin blocks: both window ends in one block, one end
+    # per block, and (T = 3000 K, P = 1e-3: W4 cores of ~300 bins) a core and a window end in
+    # the same block, or a core that fills the whole window (cutoff 0.3).
+    ll = voigt_lines
+    P = 1e-3
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 5990.0) & (ll.nu < 6210.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(6000.0, 6200.0, 40000)
+    gam = L.gamma_at(P)
+    print(f"voigt mixed short windows T={T} cutoff={cutoff} lines={L.n}", flush=True)
+    out, _ = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                         cutoff=cutoff)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=cutoff)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(out, ref)
+    print(f"  mixed max rel err {err:.3e}", flush=True)
+    assert err <= MIXED_TOL
+
+
 @pytest.mark.parametrize("width,kind", [(1.0, "core row in LDS, grid values from HBM"), (0.5, "FP64 fallback")])
 def test_voigt_mixed_fine_grids(voigt_lines, width, kind):
     # 4000 bins over `width` cm^-1 at 10000 cm^-1, T = 3000 K: the W4 cores span ~2900 (5700) bins.
