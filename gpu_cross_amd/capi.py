@@ -14,6 +14,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgx.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gx.h")
+HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "gx_ingest.h")]
 
 GX_OK, GX_INVALID_ARG, GX_OUT_OF_MEMORY, GX_NO_DEVICE, GX_DEVICE_ERROR, GX_BUSY, GX_NOT_FOUND = range(7)
 DOPPLER, VOIGT = 1, 2
@@ -90,10 +91,12 @@ def build() -> str:
 
 
 def declared_symbols() -> list:
-    """Every function name declared in include/gx.h."""
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(gx_\w+)\s*\(", text, flags=re.M)))
+    """Every function name declared in the headers under include/ (gx.h, gx_ingest.h)."""
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(gx_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
 
 
 _lib = None
@@ -145,6 +148,21 @@ def load():
     _sig(L, "gx_server_cancel", ctypes.c_int, [P, ctypes.c_uint64])
     _sig(L, "gx_server_stats", ctypes.c_int, [P, ctypes.POINTER(ServerSummary)])
     _sig(L, "gx_server_destroy", ctypes.c_int, [P])
+    sz, szp = ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)
+    _sig(L, "gx_partition", ctypes.c_int, [_dp, _ip, sz, ctypes.c_double, _dp])
+    _sig(L, "gx_states_count", ctypes.c_int, [P, sz, szp])
+    _sig(L, "gx_states_parse", ctypes.c_int, [P, sz, sz, ctypes.c_int, _ip, _dp, _ip, _ip])
+    _sig(L, "gx_ingest_create", ctypes.c_int, [ctypes.c_int, sz, _dp, _ip, _ip, ctypes.POINTER(P)])
+    _sig(L, "gx_ingest_destroy", None, [P])
+    _sig(L, "gx_ingest_set_broadening", ctypes.c_int, [P, P, sz, ctypes.c_double])
+    _sig(L, "gx_ingest_set_range", ctypes.c_int, [P, ctypes.c_double, ctypes.c_double])
+    _sig(L, "gx_ingest_begin_file", ctypes.c_int, [P])
+    _sig(L, "gx_ingest_feed", ctypes.c_int, [P, P, sz, ctypes.c_int])
+    _sig(L, "gx_ingest_count", ctypes.c_int, [P, szp, szp])
+    _sig(L, "gx_ingest_download", ctypes.c_int, [P, sz, _dp, _dp, _dp, _ip, _dp, _dp])
+    _sig(L, "gx_ingest_clear", ctypes.c_int, [P])
+    _sig(L, "gx_ingest_last_feed", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float), szp, szp])
+    _sig(L, "gx_format_trans", ctypes.c_int, [sz, _ip, _ip, _dp, P, sz, szp])
     _sig(L, "gx_last_error", ctypes.c_char_p, [])
     _sig(L, "gx_version", ctypes.c_char_p, [])
     _lib = L
