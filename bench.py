@@ -32,6 +32,10 @@ FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compe
 PROFILED_TRAFFIC = {"table_48": 4.450e12}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+# ExoMol reader: algorithmic bytes per .trans record = the text read (37 B of
+# "%12d %12d %10.4E\n") + the line written (E'', nu, A, gamma, n f64 + g i32 = 44 B)
+INGEST_BYTES_PER_RECORD = 37 + 44
 
 
 def log(msg):
@@ -126,14 +130,14 @@ def bench_table(args, wl):
     """48-point (T,P) opacity table over 1e8 resident lines: one graph replay per step."""
     from gpu_cross_amd import capi
     from gpu_cross_amd.synth import workload
+    from gpu_cross_amd.ingest import partition
     from gpu_cross_amd.table import ResidentLines, Table
-    from oracle import pyoracle as po
 
     t0 = time.time()
     ll = workload("table_48").sorted_by_nu()
     log(f"generated {ll.n} lines in {time.time() - t0:.1f} s")
     start, end, M = wl["grid"]
-    pts = [(T, P, po.partition(ll.state_e, ll.state_g, T)) for T in wl["temperatures"] for P in wl["pressures"]]
+    pts = [(T, P, partition(ll.state_e, ll.state_g, T)) for T in wl["temperatures"] for P in wl["pressures"]]
     t1 = time.time()
     lines = ResidentLines.from_linelist(ll)
     log(f"uploaded {len(lines)} lines in {time.time() - t1:.1f} s")
@@ -241,8 +245,8 @@ def bench_serving(args, wl):
     from gpu_cross_amd import capi
     from gpu_cross_amd.server import Server
     from gpu_cross_amd.synth import workload
+    from gpu_cross_amd.ingest import partition
     from gpu_cross_amd.table import ResidentLines
-    from oracle import pyoracle as po
 
     t0 = time.time()
     ll = workload("serving_3e8")
@@ -251,7 +255,7 @@ def bench_serving(args, wl):
     lines = ResidentLines.from_linelist(ll)           # sorted on the device
     log(f"uploaded + sorted on device in {time.time() - t0:.1f} s")
     qcache = {}
-    Qfun = lambda T: qcache.setdefault(T, po.partition(ll.state_e, ll.state_g, T))
+    Qfun = lambda T: qcache.setdefault(T, partition(ll.state_e, ll.state_g, T))
     submit = lambda srv, r: srv.submit(r["T"], r["P"], r["Q"], r["start"], r["end"], r["npoints"],
                                        profile=r["profile"], precision=r["precision"], cutoff=r["cutoff"],
                                        doppler_widths=r["doppler_widths"])
@@ -312,16 +316,29 @@ def bench_serving(args, wl):
     print(json.dumps(res), flush=True)
 
 
+def cpu_baseline_doppler(ll, T, Q, freq, d):
+    """The CPU baseline of the Doppler workload: the oracle (C restatement of the reference's
+    OpenMPManager) on the whole cross-section with the reference's 100 cm^-1 window, up to 16
+    threads; its grid is also the checker of the GPU result."""
+    from oracle import pyoracle as po
+
+    t2 = time.perf_counter()
+    c = po.consts(po.DOPPLER, T, Q, d, ll.mass, 1.0, 296.0)
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    ref = po.cross_section(c, ll.elow, ll.nu, ll.aif, ll.gns, None, None, freq, freq.size, 0, cores)
+    return ref, time.perf_counter() - t2, cores
+
+
 def bench_doppler(args, wl):
     """Doppler cross-section of 1e5 lines on a 1e6-bin grid (25-HWHM per-line windows), FP64."""
     from gpu_cross_amd import capi
+    from gpu_cross_amd.ingest import partition
     from gpu_cross_amd.manager import Mi355xManager, make_grid
     from gpu_cross_amd.synth import workload
-    from oracle import pyoracle as po
 
     ll = workload("doppler_1e5").sorted_by_nu()
     T = wl["T"]
-    Q = po.partition(ll.state_e, ll.state_g, T)
+    Q = partition(ll.state_e, ll.state_g, T)
     start, end, M = wl["grid"]
     freq, d = make_grid(start, end, M)
     m = Mi355xManager(capi.DOPPLER, doppler_widths=wl["doppler_widths"])
@@ -343,12 +360,7 @@ def bench_doppler(args, wl):
     m.TransferResults(None, out, M)
     m.close()
     out /= args.steps
-    # the oracle with the reference's 100 cm^-1 window on the same lines (full, it is small)
-    t2 = time.perf_counter()
-    c = po.consts(po.DOPPLER, T, Q, d, ll.mass, 1.0, 296.0)
-    cores = min(len(os.sched_getaffinity(0)), 16)
-    ref = po.cross_section(c, ll.elow, ll.nu, ll.aif, ll.gns, None, None, freq, M, 0, cores)
-    cpu_s = time.perf_counter() - t2
+    ref, cpu_s, cores = cpu_baseline_doppler(ll, T, Q, freq, d)
     mask = ref >= 1e-6 * ref.max()
     rel = np.abs(out[mask] - ref[mask]) / ref[mask]
     res = {"metric": "Doppler cross-sections/s (FP64)", "value": args.steps / wall, "unit": "xsec/s", "n_gpus": 1,
@@ -363,13 +375,86 @@ def bench_doppler(args, wl):
     print(json.dumps(res), flush=True)
 
 
+def cpu_baseline_ingest(arrays, broad, wl, trans, nrec, target_s=10.0):
+    """The CPU baseline of the ExoMol reader: the oracle's restatement of the reference's
+    read loop (one std::istream over the text, ExomolStateReader.cpp:145, one thread as the
+    reference) on the file's first records, about target_s seconds of work."""
+    from oracle import pyoracle as po
+
+    start, end, _ = wl["grid"]
+    lo, hi = start - wl["cutoff"], end + wl["cutoff"]
+    st = dict(energy=arrays["energy"], gtot=arrays["gtot"], J=arrays["J"])
+    probe = 200_000
+    t0 = time.perf_counter()
+    po.read_trans(st, broad, wl["pressure"], lo, hi, [bytes(trans[:37 * probe])])
+    per = (time.perf_counter() - t0) / probe
+    n = int(min(nrec, max(probe, target_s / per)))
+    t0 = time.perf_counter()
+    _, rec = po.read_trans(st, broad, wl["pressure"], lo, hi, [bytes(trans[:37 * n])])
+    dt = time.perf_counter() - t0
+    return {"value": rec / dt, "unit": "records/s", "cores": 1, "kind": "port",
+            "sample": f"the first {rec} records of the same .trans text ({dt:.1f} s), the oracle's istream "
+                      "read loop with the same states, broadener and range"}
+
+
+def bench_ingest(args, wl):
+    """ExoMol reader: 1e8 .trans records (3.7 GB of text) over 1e6 states with a broadener,
+    read for a 0-12000 cm^-1 grid with a 25 cm^-1 cutoff; a step reads the whole file."""
+    from gpu_cross_amd import ingest
+    from gpu_cross_amd.synth import make_exomol
+
+    t0 = time.time()
+    st_text, trans, broad, arrays = make_exomol(wl["seed"], wl["n_states"], wl["e_max"], wl["n_lines"],
+                                                wl["de_max"])
+    log(f"generated {wl['n_states']} states and {wl['n_lines']} records ({trans.size / 1e9:.2f} GB) "
+        f"in {time.time() - t0:.1f} s")
+    t0 = time.perf_counter()
+    states = ingest.parse_states(st_text)
+    states_s = time.perf_counter() - t0
+    start, end, _ = wl["grid"]
+    walls, kms = [], []
+    with ingest.ExomolReader(states, pressure=wl["pressure"], broadener=broad) as rd:
+        rd.set_range(start - wl["cutoff"], end + wl["cutoff"])
+        for _ in range(max(args.warmup, 1)):
+            rd.clear()
+            rd.read_file(trans)
+        for _ in range(args.steps):
+            rd.clear()
+            t1 = time.perf_counter()
+            ms, nbytes, nrec = rd.read_file(trans)
+            walls.append(time.perf_counter() - t1)
+            kms.append(ms / 1e3)
+            log(f"read step: {walls[-1] * 1e3:.1f} ms wall (text from host memory), device {ms:.1f} ms, "
+                f"{nrec} records")
+        nkept, nread = rd.count()
+    dev_s = float(np.mean(kms))
+    wall_s = float(np.mean(walls))
+    achieved = INGEST_BYTES_PER_RECORD * nread / dev_s / 1e9
+    res = {"metric": "ExoMol .trans records read per second (GPU reader, text in HBM)", "value": nread / dev_s,
+           "unit": "records/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dev_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "u8", "data": "synthetic",
+           "config": {"workload": f"ingest_1e8: {wl['n_lines']} records ({trans.size} bytes of .trans text), "
+                                  f"{wl['n_states']} states, broadener, range {start - wl['cutoff']}-"
+                                  f"{end + wl['cutoff']} cm^-1", "records": nread, "lines_kept": nkept},
+           "end_to_end_records_per_s": nread / wall_s,
+           "states_parse_s": states_s,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": achieved / (PEAK_HBM / 1e9), "traffic": PROFILED_TRAFFIC.get("ingest_1e8"),
+                        "kernel": "ExoMol reader passes (k_tok_count, k_tok_emit, k_records, k_keep_count, "
+                                  "k_keep_emit)", "bytes_per_record": INGEST_BYTES_PER_RECORD}}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_ingest(arrays, broad, wl, trans, nread)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="table_48",
-                    choices=["table_48", "voigt_1e8", "voigt_1e7", "serving_3e8", "doppler_1e5"])
+                    choices=["table_48", "voigt_1e8", "voigt_1e7", "serving_3e8", "doppler_1e5", "ingest_1e8"])
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per load level")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -386,8 +471,8 @@ def main():
 
     from gpu_cross_amd import capi
     from gpu_cross_amd.manager import Mi355xManager, make_grid
+    from gpu_cross_amd.ingest import partition
     from gpu_cross_amd.synth import WORKLOADS, workload
-    from oracle import pyoracle as po
 
     wl = WORKLOADS[args.workload]
     if args.workload == "table_48":
@@ -396,11 +481,13 @@ def main():
         return bench_serving(args, wl)
     if args.workload == "doppler_1e5":
         return bench_doppler(args, wl)
+    if args.workload == "ingest_1e8":
+        return bench_ingest(args, wl)
     t0 = time.time()
     ll = workload(args.workload).sorted_by_nu()
     log(f"generated {ll.n} lines in {time.time() - t0:.1f} s")
     T, P = wl["T"], wl["P"]
-    Q = po.partition(ll.state_e, ll.state_g, T)
+    Q = partition(ll.state_e, ll.state_g, T)
     start, end, M = wl["grid"]
     freq, d = make_grid(start, end, M)
     gam = ll.gamma_at(P)

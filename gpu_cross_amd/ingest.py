@@ -157,24 +157,35 @@ class ExomolReader:
         capi.check(capi.load().gx_ingest_last_feed(self._h, ctypes.byref(ms), ctypes.byref(nb), ctypes.byref(nr)))
         return ms.value, nb.value, nr.value
 
-    def read_file(self, src: Union[str, os.PathLike, bytes], block: int = _BLOCK) -> None:
-        """One .trans file (path, .bz2 path or bytes), fed in blocks."""
+    def read_file(self, src: Union[str, os.PathLike, bytes], block: int = _BLOCK):
+        """One .trans file (path, .bz2 path, bytes or uint8 array), fed in blocks.
+        Returns (device ms, text bytes, records read) summed over the feeds."""
         self.begin_file()
+        tot = [0.0, 0, 0]
+
+        def feed(piece, eof):
+            self.feed(piece, eof)
+            ms, nb, nr = self.last_feed()
+            tot[0] += ms
+            tot[1] += nb
+            tot[2] += nr
+
         if isinstance(src, (bytes, bytearray, memoryview, np.ndarray)):
             data = _text(src)
             for a in range(0, max(data.size, 1), block):
-                self.feed(data[a:a + block], a + block >= data.size)
-            return
+                feed(data[a:a + block], a + block >= data.size)
+            return tuple(tot)
         p = os.fspath(src)
         opener = bz2.open if p.endswith(".bz2") else open
         with opener(p, "rb") as f:
             chunk = f.read(block)
             while True:
                 nxt = f.read(block)
-                self.feed(chunk, not nxt)
+                feed(chunk, not nxt)
                 if not nxt:
                     break
                 chunk = nxt
+        return tuple(tot)
 
     def read_trans(self, files: Iterable[Union[str, os.PathLike, bytes]], block: int = _BLOCK) -> Lines:
         for f in files:

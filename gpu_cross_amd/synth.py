@@ -100,6 +100,53 @@ def make_lines(seed: int, n_states: int, e_max: float, n_lines: int, de_max: flo
                     np.ascontiguousarray(aif), np.ascontiguousarray(gns, dtype=np.int32), gamma_ref, nexp)
 
 
+def make_exomol(seed: int, n_states: int, e_max: float, n_lines: int, de_max: float):
+    """Seeded ExoMol-format texts (the reference's input files, ExomolStateReader.cpp:130-167):
+
+    * .states: "%12d %12.6f %6d %7d" = id, E, g_tot, J, energies sorted;
+    * .trans:  "%12d %12d %10.4E" = upper id, lower id, A, ordered by nu as ExoMol's
+      wavenumber-range files are (the reader's `break` assumes it, VoigtProfile.cpp:67);
+    * .broad:  "a1 gamma n J'' J'" for |J' - J''| <= 1 and "a0 gamma n J''" per J''.
+
+    Returns (states text bytes, trans text as a uint8 array, broad text bytes, arrays) with
+    arrays = dict(energy, gtot, J, id_f, id_i, aif)."""
+    from .ingest import format_trans
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    e = np.sort(rng.uniform(0.0, e_max, n_states))
+    j = rng.integers(0, 41, n_states)
+    gtot = ((2 * j + 1) * np.where(rng.integers(0, 2, n_states) == 0, 1, 3)).astype(np.int32)
+    up = np.empty(n_lines, dtype=np.int64)
+    lo = np.empty(n_lines, dtype=np.int64)
+    filled = 0
+    while filled < n_lines:
+        want = n_lines - filled
+        batch = int(min(max(2.6 * want + 1024, 4096), 1 << 25))
+        u = rng.integers(0, n_states, batch)
+        l = rng.integers(0, n_states, batch)
+        d = e[u] - e[l]
+        keep = np.nonzero((d > 0.0) & (d <= de_max))[0][:want]
+        k = keep.shape[0]
+        up[filled:filled + k] = u[keep]
+        lo[filled:filled + k] = l[keep]
+        filled += k
+    order = np.argsort(e[up] - e[lo], kind="stable")
+    id_f = (up[order] + 1).astype(np.int32)
+    id_i = (lo[order] + 1).astype(np.int32)
+    del up, lo, order
+    aif = 10.0 ** rng.uniform(-6.0, 2.0, n_lines)
+    states = "".join(f"{k + 1:12d} {e[k]:12.6f} {gtot[k]:6d} {j[k]:7d}\n" for k in range(n_states)).encode()
+    trans = format_trans(id_f, id_i, aif)
+    broad = []
+    for ji in range(41):
+        broad.append(f"a0 {rng.uniform(0.05, 0.12):.4f} {rng.uniform(0.5, 0.8):.3f} {ji}\n")
+        for jf in (ji - 1, ji, ji + 1):
+            if 0 <= jf <= 40:
+                broad.append(f"a1 {rng.uniform(0.05, 0.12):.4f} {rng.uniform(0.5, 0.8):.3f} {ji} {jf}\n")
+    arrays = dict(energy=e, gtot=gtot, J=j.astype(np.int32), id_f=id_f, id_i=id_i, aif=aif)
+    return states, trans, "".join(broad).encode(), arrays
+
+
 # Workload shapes (one seed each).  Grid = (start, end, bins); the reference's
 # grid is f_i = start + i*(end-start)/bins (BaseProfile.cpp:10,19).
 WORKLOADS = {
@@ -118,6 +165,9 @@ WORKLOADS = {
                      grid=(0.0, 12000.0, 1_200_000), cutoff=25.0),
     "serving_3e8": dict(seed=5, mass=17.03, n_states=3_000_000, e_max=20000.0, n_lines=300_000_000,
                         de_max=12000.0, voigt=True),
+    # ExoMol reader: .trans text of 1e8 records over 1e6 states, read for a 0-12000 cm^-1 grid
+    "ingest_1e8": dict(seed=7, mass=17.03, n_states=1_000_000, e_max=20000.0, n_lines=100_000_000,
+                       de_max=12000.0, voigt=True, pressure=1.0, grid=(0.0, 12000.0, 1_200_000), cutoff=25.0),
 }
 
 

@@ -198,3 +198,55 @@ def run_reference_driver(c: OrcConsts, elow, nu, aif, gns, gamma, nexp, start_nu
     if chunk:
         perform(chunk)
     return out
+
+
+# --- ExoMol reader restatement (ingest_oracle.cpp) --------------------------------------
+
+_ingest_lib = None
+
+
+def ingest_lib():
+    global _ingest_lib
+    if _ingest_lib is None:
+        path = os.path.join(HERE, "liboracle_ingest.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle_ingest.so"], check=True)
+        L = ctypes.CDLL(path)
+        L.orc_states.restype = ctypes.c_long
+        L.orc_ingest.restype = ctypes.c_long
+        _ingest_lib = L
+    return _ingest_lib
+
+
+def read_states(text: bytes) -> dict:
+    """.states text read as initialize_states reads it (exomol_functions.cpp:8-48)."""
+    L = ingest_lib()
+    n = L.orc_states(text, ctypes.c_long(len(text)), ctypes.c_long(0), None, None, None, None)
+    out = dict(id=np.zeros(n, np.int32), energy=np.zeros(n), gtot=np.zeros(n, np.int32), J=np.zeros(n, np.int32))
+    L.orc_states(text, ctypes.c_long(len(text)), ctypes.c_long(n), _i(out["id"]), _d(out["energy"]),
+                 _i(out["gtot"]), _i(out["J"]))
+    return out
+
+
+def read_trans(states: dict, broad: Optional[bytes], pressure: float, lo: float, hi: float, files,
+               use_broad: bool = True):
+    """The reference's read loop over .trans texts (ReadNextState + GetGammaN + the filter of
+    VoigtProfile.cpp:56-68).  Returns ([E'', nu, A, g, gamma, n] arrays, records read)."""
+    L = ingest_lib()
+    e = np.ascontiguousarray(states["energy"], np.float64)
+    g = np.ascontiguousarray(states["gtot"], np.int32)
+    J = np.ascontiguousarray(states["J"], np.int32)
+    files = [bytes(f) for f in files]
+    cap = sum(len(f) for f in files) // 5 + 16
+    arrs = [np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap), np.zeros(cap)]
+    fptrs = (ctypes.c_char_p * len(files))(*files)
+    lens = (ctypes.c_long * len(files))(*[len(f) for f in files])
+    rec = ctypes.c_long(0)
+    b = broad or b""
+    n = L.orc_ingest(_d(e), _i(g), _i(J), ctypes.c_long(e.size), b, ctypes.c_long(len(b)),
+                     ctypes.c_int(1 if use_broad else 0), ctypes.c_double(pressure), ctypes.c_double(lo),
+                     ctypes.c_double(hi), fptrs, lens, ctypes.c_int(len(files)), ctypes.c_long(cap),
+                     _d(arrs[0]), _d(arrs[1]), _d(arrs[2]), _i(arrs[3]), _d(arrs[4]), _d(arrs[5]), ctypes.byref(rec))
+    if n < 0:
+        raise ValueError(f"oracle reader: state id out of range ({n})")
+    return [a[:n] for a in arrs], rec.value

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from gpu_cross_amd import capi, ingest
+from oracle import pyoracle as po
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -78,44 +79,13 @@ EXP_LINES = [
 EXP_RECORDS = 7 + 2 + 2
 
 
-def _orc():
-    path = os.path.join(ROOT, "oracle", "liboracle_ingest.so")
-    if not os.path.exists(path):
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle_ingest.so"], check=True)
-    L = ctypes.CDLL(path)
-    L.orc_states.restype = ctypes.c_long
-    L.orc_ingest.restype = ctypes.c_long
-    return L
-
-
 def oracle_states(text: bytes):
-    L = _orc()
-    n = L.orc_states(text, ctypes.c_long(len(text)), ctypes.c_long(0), None, None, None, None)
-    out = dict(id=np.zeros(n, np.int32), energy=np.zeros(n), gtot=np.zeros(n, np.int32), J=np.zeros(n, np.int32))
-    L.orc_states(text, ctypes.c_long(len(text)), ctypes.c_long(n), capi.iptr(out["id"]), capi.dptr(out["energy"]),
-                 capi.iptr(out["gtot"]), capi.iptr(out["J"]))
-    return out
+    return po.read_states(text)
 
 
 def oracle_ingest(states, broad, pressure, lo, hi, files, use_broad=True):
     """The oracle's read loop; returns (E'', nu, A, g, gamma, n) arrays and the records read."""
-    L = _orc()
-    e = np.ascontiguousarray(states["energy"], np.float64)
-    g = np.ascontiguousarray(states["gtot"], np.int32)
-    J = np.ascontiguousarray(states["J"], np.int32)
-    cap = sum(len(f) for f in files) // 5 + 16
-    arrs = [np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap), np.zeros(cap)]
-    fptrs = (ctypes.c_char_p * len(files))(*files)
-    lens = (ctypes.c_long * len(files))(*[len(f) for f in files])
-    rec = ctypes.c_long(0)
-    b = broad or b""
-    n = L.orc_ingest(capi.dptr(e), capi.iptr(g), capi.iptr(J), ctypes.c_long(e.size), b, ctypes.c_long(len(b)),
-                     ctypes.c_int(1 if use_broad else 0), ctypes.c_double(pressure), ctypes.c_double(lo),
-                     ctypes.c_double(hi), fptrs, lens, ctypes.c_int(len(files)), ctypes.c_long(cap),
-                     capi.dptr(arrs[0]), capi.dptr(arrs[1]), capi.dptr(arrs[2]), capi.iptr(arrs[3]),
-                     capi.dptr(arrs[4]), capi.dptr(arrs[5]), ctypes.byref(rec))
-    assert n >= 0, f"oracle error {n}"
-    return [a[:n] for a in arrs], rec.value
+    return po.read_trans(states, broad, pressure, lo, hi, files, use_broad)
 
 
 def test_oracle_states_known_answer():
@@ -181,7 +151,6 @@ def test_product_states_parser_matches_oracle_on_odd_text():
 
 
 def test_partition_closed_form_and_oracle():
-    from oracle import pyoracle as po
     beta = 6.6260693e-27 * 2.99792458e10 / (1.380658e-16 * 300.0)
     q = ingest.partition(np.array([0.0, 100.0]), np.array([1, 3]), 300.0)
     assert q == 1.0 + 3 * math.exp(-beta * 100.0)
