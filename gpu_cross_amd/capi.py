@@ -14,7 +14,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgx.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gx.h")
-HEADERS = [HEADER, os.path.join(os.path.dirname(HERE), "include", "gx_ingest.h")]
+HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", h) for h in ("gx_ingest.h", "gx_output.h")]
 
 GX_OK, GX_INVALID_ARG, GX_OUT_OF_MEMORY, GX_NO_DEVICE, GX_DEVICE_ERROR, GX_BUSY, GX_NOT_FOUND = range(7)
 DOPPLER, VOIGT = 1, 2
@@ -80,6 +80,13 @@ class ServerSummary(ctypes.Structure):
                 ("evals", ctypes.c_longlong), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("busy_s", ctypes.c_double),
                 ("peak_bytes", ctypes.c_size_t)]
+
+
+class OpacityInfo(ctypes.Structure):
+    _fields_ = [("npoints", ctypes.c_uint64), ("npts", ctypes.c_uint64), ("start_nu", ctypes.c_double),
+                ("end_nu", ctypes.c_double), ("dfreq", ctypes.c_double), ("profile", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("mean_mass", ctypes.c_double), ("ref_temp", ctypes.c_double),
+                ("cutoff", ctypes.c_double), ("data_offset", ctypes.c_uint64)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
@@ -163,6 +170,12 @@ def load():
     _sig(L, "gx_ingest_clear", ctypes.c_int, [P])
     _sig(L, "gx_ingest_last_feed", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float), szp, szp])
     _sig(L, "gx_format_trans", ctypes.c_int, [sz, _ip, _ip, _dp, P, sz, szp])
+    _sig(L, "gx_xsec_text", ctypes.c_int, [_dp, _dp, ctypes.c_int, P, sz, szp])
+    _sig(L, "gx_xsec_text_write", ctypes.c_int, [ctypes.c_char_p, _dp, _dp, ctypes.c_int])
+    _sig(L, "gx_opacity_write", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(OpacityInfo), ctypes.POINTER(Point),
+                                               _dp])
+    _sig(L, "gx_opacity_read_info", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(OpacityInfo)])
+    _sig(L, "gx_opacity_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Point), sz, _dp, sz])
     _sig(L, "gx_last_error", ctypes.c_char_p, [])
     _sig(L, "gx_version", ctypes.c_char_p, [])
     _lib = L

@@ -56,6 +56,9 @@ class Table:
         self.lines = lines
         self.npoints = npoints
         self.points = list(points)
+        self.start_nu, self.end_nu = float(start_nu), float(end_nu)
+        self.profile, self.precision = profile, precision
+        self.mean_mass, self.ref_temp, self.cutoff = mean_mass, ref_temp, cutoff
         grid = capi.Grid(start_nu, end_nu, npoints)
         pts = (capi.Point * len(self.points))(*[capi.Point(float(T), float(P), float(Q)) for T, P, Q in self.points])
         opts = capi.TableOpts(profile, precision, convention, mean_mass, ref_temp, cutoff, doppler_widths,
@@ -78,6 +81,16 @@ class Table:
             out = np.empty(self.npoints, np.float64)
         check(self._L.gx_table_read(self._h, point, dptr(out), self.npoints))
         return out
+
+    def save(self, path) -> None:
+        """Write every grid of the last run to an opacity-table file (gpu_cross_amd.opacity)."""
+        from .opacity import write_table
+
+        grids = np.empty((len(self.points), self.npoints), np.float64)
+        for k in range(len(self.points)):
+            self.read(k, grids[k])
+        write_table(path, self.points, grids, self.start_nu, self.end_nu, self.profile, self.precision,
+                    self.mean_mass, self.ref_temp, self.cutoff)
 
     def shape(self):
         r, s = ctypes.c_int(), ctypes.c_int()

@@ -101,3 +101,21 @@ def test_table_grid_equals_single_point_manager(setup):
         m.TransferResults(None, out, M)
     np.testing.assert_allclose(tab.read(0), out, rtol=1e-6, atol=0)
     tab.close()
+
+
+def test_table_saved_to_an_opacity_file(setup, tmp_path):
+    from gpu_cross_amd import opacity
+    ll, sub, lines, pts = setup
+    M = 8000
+    tab = Table(lines, 2050.0, 2150.0, M, pts[:3], 17.03, cutoff=25.0, use_graph=False, autotune=False)
+    tab.run()
+    tab.wait()
+    path = tmp_path / "table.gxopac"
+    tab.save(path)
+    t = opacity.read_table(path)
+    assert t.npoints == M and t.start_nu == 2050.0 and t.end_nu == 2150.0
+    assert np.array_equal(t.points, np.array(pts[:3]))
+    for k in range(3):
+        assert np.array_equal(np.asarray(t.grids[k]), tab.read(k))
+    freq, _ = make_grid(2050.0, 2150.0, M)
+    assert np.array_equal(t.freq, freq)
