@@ -148,9 +148,10 @@ def bench_table(args, wl):
         tab.run()
         t = tab.wait()
     eager_int, eager_acc = t.intensity_ms, t.accumulate_ms
-    r1_ms, core_ms = t.region1_ms, t.core_ms
+    r1_ms, core_ms, eager_far_ms = t.region1_ms, t.core_ms, t.far_ms
     log(f"eager run: prep {eager_int:.1f} ms, accumulate {eager_acc:.1f} ms "
-        f"(region I {r1_ms:.1f} ms, W4 core pass {core_ms:.1f} ms)")
+        f"(far field {eager_far_ms:.1f} ms for {t.far_evals / max(t.evals, 1):.1%} of the pairs, region I "
+        f"{r1_ms:.1f} ms, W4 core pass {core_ms:.1f} ms)")
     walls = []
     for _ in range(args.steps):
         t1 = time.perf_counter()
@@ -160,7 +161,8 @@ def bench_table(args, wl):
         log(f"table step: {walls[-1] * 1e3:.1f} ms wall, graph {t.total_ms:.1f} ms, evals {t.evals:.4e}")
     evals = t.evals
     core_evals = t.core_evals
-    r1_evals = evals - core_evals                 # region-I evaluations (the dominant kernel's)
+    far_evals, far_ms = t.far_evals, eager_far_ms
+    r1_evals = evals - core_evals - far_evals     # region-I pairs the direct kernel evaluated
     wall = float(np.sum(walls))
     value = evals * args.steps / wall
     ms_step = wall / args.steps * 1e3
@@ -183,6 +185,8 @@ def bench_table(args, wl):
                      "flop_per_eval": FLOP_PER_EVAL},
         "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV)", "ms": core_ms, "evals": core_evals,
                       "evals_per_s": core_evals / (core_ms * 1e-3) if core_ms > 0 else None},
+        "far_field": {"kernels": "k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_far_eval (FP64 moments)",
+                      "ms": far_ms, "evals": far_evals, "share": far_evals / max(evals, 1)},
         "intensity_kernels_ms": eager_int,
     }
     mixed_grids = [tab.read(p) for p in range(len(pts))]

@@ -45,7 +45,8 @@ class Consts(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("intensity_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
                 ("evals", ctypes.c_longlong), ("lines", ctypes.c_longlong), ("region1_ms", ctypes.c_float),
-                ("core_ms", ctypes.c_float), ("core_evals", ctypes.c_longlong)]
+                ("core_ms", ctypes.c_float), ("core_evals", ctypes.c_longlong), ("far_ms", ctypes.c_float),
+                ("far_evals", ctypes.c_longlong)]
 
 
 class Grid(ctypes.Structure):

@@ -255,8 +255,12 @@ int MixedRunner::init() {
     const int r = std::atoi(env);
     if (r == 4 || r == 8) R = r;
   }
+  if (const char* env = std::getenv("GX_FAR")) far_on = std::atoi(env) != 0;
   hipError_t e = hipMalloc(&counter, sizeof(unsigned));
   if (e) return hip_fail(e, "hipMalloc(counter)");
+  if ((e = hipMalloc(&far_evals, sizeof(unsigned long long))) || (e = hipEventCreate(&far_ev[0])) ||
+      (e = hipEventCreate(&far_ev[1])))
+    return hip_fail(e, "far-field counters");
   int dev = 0;
   if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)))
     return hip_fail(e, "device attributes");
@@ -268,7 +272,26 @@ void MixedRunner::release() {
   hipFree(d_tasks); hipFree(partial); hipFree(counter); hipFree(starts); hipFree(core_part);
   d_tasks = nullptr; partial = nullptr; counter = nullptr; starts = nullptr; core_part = nullptr;
   task_cap = 0; partial_cap = 0; cus = 0; starts_cap = 0; core_cap = 0;
+  hipFree(far_geom); hipFree(far_rec); hipFree(far_rng); hipFree(far_list); hipFree(far_cnt); hipFree(far_evals);
+  far_geom = nullptr; far_rec = nullptr; far_rng = nullptr; far_list = nullptr; far_cnt = nullptr;
+  far_evals = nullptr; far_geom_cap = far_rec_cap = far_blk_cap = 0;
+  for (auto& ev : far_ev) {
+    if (ev) hipEventDestroy(ev);
+    ev = nullptr;
+  }
+  far_timed = false;
   h_tasks.clear();
+}
+
+int MixedRunner::far_time(float* ms, unsigned long long* evals) {
+  *ms = 0.0f;
+  *evals = 0;
+  if (!far_timed || !plan.far) return GX_OK;
+  hipError_t e;
+  if ((e = hipEventElapsedTime(ms, far_ev[0], far_ev[1])) ||
+      (e = hipMemcpy(evals, far_evals, sizeof(unsigned long long), hipMemcpyDeviceToHost)))
+    return hip_fail(e, "far-field timing");
+  return GX_OK;
 }
 
 double chunk_max_abs_nu(const GridView& gv) {
@@ -393,7 +416,71 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     plan.starts = starts;
     plan.core_part = core_part;
   }
+  // far field: leaves of kFarLeaf lines and their binary tree
+  plan.far = 0;
+  if (far_on && plan.nlines >= 2 * kFarLeaf && gv.points > 0) {
+    int n = (plan.nlines + kFarLeaf - 1) / kFarLeaf, lev = 0, tot = 0;
+    while (true) {
+      if (lev >= kFarMaxLevels) return fail(GX_INVALID_ARG, "mixed: too many far-field levels");
+      plan.far_off[lev] = tot;
+      plan.far_n[lev] = n;
+      tot += n;
+      ++lev;
+      if (n == 1) break;
+      n = (n + 1) / 2;
+    }
+    plan.far_nlev = lev;
+    plan.far_nclus = tot;
+    const size_t need_rec = (size_t)ntasks * tot * kFarRec;
+    const size_t need_blk = (size_t)ntasks * nblocks;
+    if ((size_t)tot > far_geom_cap || need_rec > far_rec_cap || need_blk > far_blk_cap) {
+      if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
+      if ((size_t)tot > far_geom_cap) {
+        hipFree(far_geom);
+        far_geom = nullptr;
+        far_geom_cap = 0;
+        if ((e = hipMalloc(&far_geom, sizeof(double4) * (size_t)tot))) return hip_fail(e, "hipMalloc(far geometry)");
+        far_geom_cap = tot;
+      }
+      if (need_rec > far_rec_cap) {
+        hipFree(far_rec);
+        far_rec = nullptr;
+        far_rec_cap = 0;
+        if ((e = hipMalloc(&far_rec, sizeof(double) * need_rec))) return hip_fail(e, "hipMalloc(far moments)");
+        far_rec_cap = need_rec;
+      }
+      if (need_blk > far_blk_cap) {
+        hipFree(far_rng);
+        hipFree(far_list);
+        hipFree(far_cnt);
+        far_rng = nullptr;
+        far_list = nullptr;
+        far_cnt = nullptr;
+        far_blk_cap = 0;
+        if ((e = hipMalloc(&far_rng, sizeof(int4) * need_blk)) ||
+            (e = hipMalloc(&far_list, sizeof(int) * need_blk * kFarList)) ||
+            (e = hipMalloc(&far_cnt, sizeof(int) * need_blk)))
+          return hip_fail(e, "hipMalloc(far plans)");
+        far_blk_cap = need_blk;
+      }
+    }
+    plan.far = 1;
+    plan.far_bins = bins_per_block();
+    plan.far_geom = far_geom;
+    plan.far_rec = far_rec;
+    plan.far_rng = far_rng;
+    plan.far_list = far_list;
+    plan.far_cnt = far_cnt;
+    plan.far_evals = far_evals;
+  }
   return GX_OK;
+}
+
+// Bytes of the far-field buffers for ntasks points over nlines lines and nblocks blocks.
+size_t far_field_bytes(int ntasks, long long nlines, int nblocks) {
+  const long long clusters = 2 * ((nlines + kFarLeaf - 1) / kFarLeaf) + 64;
+  return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * kFarRec * sizeof(double)) +
+         (size_t)ntasks * nblocks * (sizeof(int4) + sizeof(int) * (kFarList + 1));
 }
 
 int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie,
@@ -405,6 +492,14 @@ int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges
   plan.ib = ib;
   plan.ie = ie;
   hipError_t e;
+  far_timed = false;
+  if (plan.far) {
+    if (ev && (e = hipEventRecord(far_ev[0], s))) return hip_fail(e, "event");
+    if ((e = launch_far_setup(plan, s))) return hip_fail(e, "far-field setup launch");
+    if ((e = launch_far_eval(plan, R, s))) return hip_fail(e, "far-field launch");
+    if (ev && (e = hipEventRecord(far_ev[1], s))) return hip_fail(e, "event");
+    far_timed = ev != nullptr;
+  }
   if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");
   if ((e = launch_accum_mixed(plan, R, s))) return hip_fail(e, "mixed accumulate launch");
   if (ev && (e = hipEventRecord(ev[1], s))) return hip_fail(e, "event");
@@ -695,6 +790,8 @@ int gx_last_timing(gx_ctx* c, gx_timing* out) {
   out->region1_ms = b;
   out->core_ms = 0.0f;
   out->core_evals = 0;
+  out->far_ms = 0.0f;
+  out->far_evals = 0;
   if (c->last_mixed) {
     float r1 = 0, co = 0;
     if ((e = hipEventElapsedTime(&r1, c->ev[4], c->ev[5])) || (e = hipEventElapsedTime(&co, c->ev[5], c->ev[6])))
@@ -705,6 +802,10 @@ int gx_last_timing(gx_ctx* c, gx_timing* out) {
     out->region1_ms = r1;
     out->core_ms = co;
     out->core_evals = (long long)ce;
+    unsigned long long fe = 0;
+    const int frc = c->mix.far_time(&out->far_ms, &fe);
+    if (frc) return frc;
+    out->far_evals = (long long)fe;
   }
   return GX_OK;
 }

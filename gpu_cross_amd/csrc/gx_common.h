@@ -85,20 +85,41 @@ struct MixedRunner {
   int forced_nseg = 0;         // > 0: fixed segments per block
   bool persistent = true;      // false: one workgroup per work item, so the hardware interleaves
                                // workgroups of launches on other streams (shared device)
+  // far field of region I (gx_far.hip); GX_FAR=0 switches it off
+  bool far_on = true;
+  double4* far_geom = nullptr;
+  size_t far_geom_cap = 0;     // clusters
+  double* far_rec = nullptr;
+  size_t far_rec_cap = 0;      // doubles
+  int4* far_rng = nullptr;
+  int* far_list = nullptr;
+  int* far_cnt = nullptr;
+  size_t far_blk_cap = 0;      // (task, block) slots
+  unsigned long long* far_evals = nullptr;
+  hipEvent_t far_ev[2] = {nullptr, nullptr};
+  bool far_timed = false;      // the last launch recorded far_ev
   std::vector<MixedTask> h_tasks;
   MixedPlan plan{};
   int init();
+  // Device time of the last timed launch's far-field kernels and the line-bin pairs they
+  // took from moments (call after the stream is synchronised).
+  int far_time(float* ms, unsigned long long* evals);
   int bins_per_block() const { return mixed_bins_per_block(R); }
   // Upload the task table, size the partials and the core-pass buffers for the
   // chunk gv of the grid and nlines sorted lines, and fix the launch shape.
   // May synchronise the stream; never call it inside a graph capture.
   int prepare(const MixedTask* tasks, int ntasks, int nblocks, const GridView& gv, int nlines, hipStream_t s);
-  // Enqueue the kernels of the prepared plan (graph-capturable).  ev, when not NULL, gets
-  // 3 events: before region I, between region I and the core pass, after the core pass.
+  // Enqueue the kernels of the prepared plan (graph-capturable): the far field (moments,
+  // block plans, far sums), region I, the core pass.  ev, when not NULL, gets 3 events:
+  // before region I, between region I and the core pass, after the core pass; the far
+  // field is then timed by the runner's own events (far_time).
   int launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie, hipStream_t s,
              hipEvent_t* ev = nullptr);
   void release();
 };
+
+// HBM of the far-field buffers of a mixed launch (ntasks points, nlines lines, nblocks blocks).
+size_t far_field_bytes(int ntasks, long long nlines, int nblocks);
 
 // Core pass of the mixed path: bound on |bin - centre bin| inside any W4 core of the
 // chunk gv at Doppler width coefficient dpw, or -1 when its bin row does not fit LDS

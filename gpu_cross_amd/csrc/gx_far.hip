@@ -1,0 +1,311 @@
+// gx_far.hip -- far field of the mixed path's region I.
+//
+// W4's region-I term (|x| + y >= 15) is T c / (1/2 + T^2) with T = y - i x and c the
+// 0.5641896 of humlic (profiles.cpp:16-18).  Since T / (1/2 + T^2) = (1/(T - ia) +
+// 1/(T + ia)) / 2 with a = 1/sqrt(2), and x = (nu - f) g, y = gamma g, one line adds
+//     S g Re w_I = Re[ i kappa (1/(zeta+ - f) + 1/(zeta- - f)) ],
+//     kappa = c S / 2,  zeta+- = nu +- a/g + i gamma          (1/g = alpha, y/g = gamma)
+// to bin f: two complex poles per line.  For a cluster of lines with centre nu_c and
+// every pole within r of it, and a bin at d = f - nu_c with |d| >= r / eta,
+//     sum over the cluster = (1/d) sum_{k>=1} Q_k (r/d)^k,
+//     Q_k = r^-k sum_lines kappa (Im (zeta+ - nu_c)^k + Im (zeta- - nu_c)^k),
+// a real series in the real variable r/d; the terms beyond k = 24 add at most
+// ~25 eta^24 = 1e-6 (eta = 1/2) relative to the cluster's sum, since each line's
+// Im (zeta - nu_c)^k is gamma times a polynomial of degree k - 1.
+//
+// Clusters: leaves of 64 consecutive lines of the sorted list and their binary tree.
+// Leaf moments come from the lines (k_far_leaf), a parent's from its children's by
+// the real shift nu_c' - nu_c (k_far_up).  For each (point, bin block), k_far_plan picks
+// clusters among the lines whose windows cover the whole block (a contiguous slice of
+// the sorted list), far enough that the series converges and that no line's W4 core
+// reaches the block (so every pair is a region-I pair of the FP64 path too); the
+// region-I kernel skips those lines and k_far_eval adds their sums in FP64.
+#include <hip/hip_runtime.h>
+
+#include "gx_internal.h"
+
+namespace gx {
+namespace {
+
+constexpr double kPoleShift = 0.70710678118654752440;   // a = 1/sqrt(2)
+constexpr double kKappa = 0.5 * 0.5641896;             // c / 2, c of humlic's region I
+
+__device__ __forceinline__ double nu_if(double v) { return v == 0.0 ? 1e-6 : v; }
+
+// Cluster geometry, for every level: centre = midpoint of its lines' nu range.
+__global__ void k_far_geom(MixedPlan P, double4* __restrict__ geom) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P.far_nclus) return;
+  int lev = 0;
+  while (lev + 1 < P.far_nlev && k >= P.far_off[lev + 1]) ++lev;
+  const int idx = k - P.far_off[lev];
+  const long long l0 = (long long)idx * kFarLeaf << lev;
+  const long long l1 = min((long long)P.nlines, l0 + ((long long)kFarLeaf << lev));
+  const double lo = nu_if(P.nu[l0]), hi = nu_if(P.nu[l1 - 1]);
+  geom[k] = make_double4(0.5 * (lo + hi), lo, hi, 0.0);
+}
+
+// Leaf moments of every task: one thread per (leaf, task).
+__global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
+#pragma clang fp contract(off)
+  const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= P.far_n[0]) return;
+  const MixedTask tk = P.tasks[blockIdx.y];
+  const int l0 = leaf * kFarLeaf, l1 = min(P.nlines, l0 + kFarLeaf);
+  const double c = P.far_geom[leaf].x;
+  double r2 = 0.0, amax = 0.0;
+  for (int l = l0; l < l1; ++l) {
+    const double nu = nu_if(P.nu[l]);
+    const double al = 1.0 / tk.g[l];
+    const double s = kPoleShift * al, im = tk.y[l] * al;
+    const double dp = nu + s - c, dm = nu - s - c;
+    r2 = fmax(r2, fmax(dp * dp, dm * dm) + im * im);
+    amax = fmax(amax, al);
+  }
+  double r = sqrt(r2);
+  if (!(r > 0.0)) r = 1.0;
+  const double ir = 1.0 / r;
+  double q[kFarP];
+#pragma unroll
+  for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
+  for (int l = l0; l < l1; ++l) {
+    const double nu = nu_if(P.nu[l]);
+    const double g = tk.g[l];
+    const double al = 1.0 / g;
+    const double kap = kKappa * (double)tk.amp[l] * al;       // amp = S g
+    const double s = kPoleShift * al, im = tk.y[l] * al * ir;
+#pragma unroll
+    for (int pole = 0; pole < 2; ++pole) {
+      const double wr = (pole ? nu - s - c : nu + s - c) * ir;
+      double pr = wr, pi = im;                                 // w^1
+#pragma unroll
+      for (int k = 0; k < kFarP; ++k) {
+        q[k] += kap * pi;
+        const double nr = pr * wr - pi * im;
+        const double ni = pr * im + pi * wr;
+        pr = nr;
+        pi = ni;
+      }
+    }
+  }
+  double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
+  rec[0] = r;
+  rec[1] = amax;
+#pragma unroll
+  for (int k = 0; k < kFarP; ++k) rec[2 + k] = q[k];
+}
+
+// Parents of level lev from their two children: centre shift a = (c_child - c) / r and
+// scale b = r_child / r, Q_k = sum_m C(k, m) a^(k-m) b^m Q_m(child).
+__global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.far_n[lev]) return;
+  const size_t tbase = (size_t)blockIdx.y * P.far_nclus;
+  const int self = P.far_off[lev] + i;
+  const double c = P.far_geom[self].x;
+  const int nch = min(2, P.far_n[lev - 1] - 2 * i);
+  double r = 0.0, amax = 0.0;
+  for (int h = 0; h < nch; ++h) {
+    const int ch = P.far_off[lev - 1] + 2 * i + h;
+    const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+    r = fmax(r, fabs(P.far_geom[ch].x - c) + cr[0]);
+    amax = fmax(amax, cr[1]);
+  }
+  if (!(r > 0.0)) r = 1.0;
+  double q[kFarP];
+#pragma unroll
+  for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
+  for (int h = 0; h < nch; ++h) {
+    const int ch = P.far_off[lev - 1] + 2 * i + h;
+    const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+    const double a = (P.far_geom[ch].x - c) / r, b = cr[0] / r;
+    double pa[kFarP], qb[kFarP];                               // a^j, b^m Q_m
+    pa[0] = 1.0;
+    double bm = b;
+#pragma unroll
+    for (int j = 0; j < kFarP; ++j) {
+      if (j) pa[j] = pa[j - 1] * a;
+      qb[j] = bm * cr[2 + j];
+      bm *= b;
+    }
+    // Q_k (k = 1..P) += sum_{m=1..k} C(k, m) a^(k-m) b^m Q_m
+#pragma unroll
+    for (int k = 1; k <= kFarP; ++k) {
+      double cb = k;                                           // C(k, 1)
+      double acc = 0.0;
+#pragma unroll
+      for (int m = 1; m <= k; ++m) {
+        acc += cb * pa[k - m] * qb[m - 1];
+        cb = cb * (double)(k - m) / (double)(m + 1);
+      }
+      q[k - 1] += acc;
+    }
+  }
+  double* rec = P.far_rec + (tbase + self) * kFarRec;
+  rec[0] = r;
+  rec[1] = amax;
+#pragma unroll
+  for (int k = 0; k < kFarP; ++k) rec[2 + k] = q[k];
+}
+
+__global__ void k_far_reset(unsigned long long* c) { *c = 0ull; }
+
+// First index in [a, z) with v[index] >= key (v non-decreasing there).
+__device__ __forceinline__ int lower_bound_int(const int* __restrict__ v, int a, int z, int key) {
+  while (a < z) {
+    const int m = a + (z - a) / 2;
+    if (v[m] < key) a = m + 1;
+    else z = m;
+  }
+  return a;
+}
+
+// The far clusters of one (task, block): one thread each.
+__global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.nblocks) return;
+  const int task = blockIdx.y;
+  const size_t slot = (size_t)task * P.nblocks + b;
+  int4 rng = make_int4(0, 0, 0, 0);
+  int cnt = 0;
+  unsigned long long far_lines = 0;
+  const GridView gv = P.gv;
+  const int w0 = gv.start_idx + b * P.far_bins;
+  const int w1 = w0 + P.far_bins;
+  const int2 rg = P.ranges[b];
+  if (w1 <= gv.start_idx + gv.points && rg.y > rg.x) {
+    const double f_lo = gv.freq[w0], f_hi = gv.freq[w1 - 1];
+    // lines whose windows cover the whole block: ie >= w1 and ib <= w0 (ib, ie rise with nu)
+    const int L1 = lower_bound_int(P.ie, rg.x, rg.y, w1);
+    const int L2 = lower_bound_int(P.ib, rg.x, rg.y, w0 + 1);
+    const int leafL = (L1 + kFarLeaf - 1) / kFarLeaf, leafR = L2 / kFarLeaf;
+    const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+    int* list = P.far_list + slot * kFarList;
+    // a cluster serves the block when its poles are within eta of the distance to the
+    // block and its lines are at least 15 Doppler widths from every bin (no W4 core)
+    auto usable = [&](int lev, int idx) -> bool {
+      const int k = P.far_off[lev] + idx;
+      const double4 gm = P.far_geom[k];
+      const double* rc = recs + (size_t)k * kFarRec;
+      double dist, edge;
+      if (gm.z < f_lo) {
+        dist = f_lo - gm.x;
+        edge = f_lo - gm.z;
+      } else if (gm.y > f_hi) {
+        dist = gm.x - f_hi;
+        edge = gm.y - f_hi;
+      } else {
+        return false;
+      }
+      return rc[0] <= kFarEta * dist && edge >= 15.001 * rc[1];
+    };
+    int i = leafL;
+    if (leafL < leafR) {
+      // from the left end of the slice towards the block
+      while (i < leafR && cnt < kFarList) {
+        int lev = min(P.far_nlev - 1, i ? __builtin_ctz(i) : 31);
+        bool took = false;
+        for (; lev >= 0; --lev) {
+          if ((i + (1 << lev)) > leafR) continue;
+          if (usable(lev, i >> lev)) {
+            list[cnt++] = (lev << 24) | (i >> lev);
+            i += 1 << lev;
+            took = true;
+            break;
+          }
+        }
+        if (!took) break;
+      }
+      int j = leafR;
+      while (j > i && cnt < kFarList) {
+        int lev = min(P.far_nlev - 1, j ? __builtin_ctz(j) : 31);
+        bool took = false;
+        for (; lev >= 0; --lev) {
+          if (j - (1 << lev) < i) continue;
+          if (usable(lev, (j - (1 << lev)) >> lev)) {
+            list[cnt++] = (lev << 24) | ((j - (1 << lev)) >> lev);
+            j -= 1 << lev;
+            took = true;
+            break;
+          }
+        }
+        if (!took) break;
+      }
+      rng = make_int4(leafL * kFarLeaf, i * kFarLeaf, j * kFarLeaf, leafR * kFarLeaf);
+      far_lines = (unsigned long long)(rng.y - rng.x) + (unsigned long long)(rng.w - rng.z);
+    }
+  }
+  P.far_rng[slot] = rng;
+  P.far_cnt[slot] = cnt;
+  if (far_lines) atomicAdd(P.far_evals, far_lines * (unsigned long long)P.far_bins);
+}
+
+// Far-field sums of one (task, block) per wave, added to the task's grid.
+template <int R>
+__global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= P.nblocks) return;
+  const int task = blockIdx.y;
+  const size_t slot = (size_t)task * P.nblocks + b;
+  const int cnt = P.far_cnt[slot];
+  if (cnt == 0) return;
+  const MixedTask tk = P.tasks[task];
+  const GridView gv = P.gv;
+  const int w0 = gv.start_idx + b * 64 * R;
+  const int* list = P.far_list + slot * kFarList;
+  const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  double f[R], acc[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    f[q] = gv.freq[w0 + lane + 64 * q];       // far clusters are planned for whole blocks only
+    acc[q] = 0.0;
+  }
+  for (int e = 0; e < cnt; ++e) {
+    const int ent = list[e];
+    const int lev = ent >> 24, idx = ent & 0xffffff;
+    const int k = P.far_off[lev] + idx;
+    const double c = P.far_geom[k].x;
+    const double* rc = recs + (size_t)k * kFarRec;
+    const double r = rc[0];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const double id = 1.0 / (f[q] - c);
+      const double t = r * id;
+      double s = rc[2 + kFarP - 1];
+#pragma unroll
+      for (int m = kFarP - 2; m >= 0; --m) s = s * t + rc[2 + m];
+      acc[q] += s * t * id;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) tk.out[w0 + lane + 64 * q] += acc[q];
+}
+
+}  // namespace
+
+hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
+  if (!plan.far || plan.ntasks <= 0) return hipSuccess;
+  k_far_geom<<<(plan.far_nclus + 255) / 256, 256, 0, s>>>(plan, plan.far_geom);
+  k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
+  k_far_leaf<<<dim3((plan.far_n[0] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
+  for (int lev = 1; lev < plan.far_nlev; ++lev)
+    k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
+  k_far_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
+  return hipGetLastError();
+}
+
+hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
+  if (!plan.far || plan.ntasks <= 0) return hipSuccess;
+  const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
+  if (R == 4)
+    k_far_eval<4><<<grid, 256, 0, s>>>(plan);
+  else
+    k_far_eval<8><<<grid, 256, 0, s>>>(plan);
+  return hipGetLastError();
+}
+
+}  // namespace gx

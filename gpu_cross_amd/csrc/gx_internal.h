@@ -84,6 +84,14 @@ struct MixedTask {
   double* out;
 };
 
+// Far field of region I: moments per cluster, leaf size, list length, levels.
+constexpr int kFarP = 24;            // moments Q_1..Q_24 (truncation ~ 25 eta^24 relative)
+constexpr double kFarEta = 0.5;      // a cluster of radius r serves bins at distance >= r / eta
+constexpr int kFarLeaf = 64;         // lines per leaf
+constexpr int kFarRec = 2 + kFarP;   // radius, max 1/g, moments
+constexpr int kFarList = 128;        // clusters per block at most
+constexpr int kFarMaxLevels = 32;
+
 // A mixed-precision launch over ntasks points sharing one grid, one sorted
 // line list and one set of block ranges.  Work items are (task, block, segment).
 struct MixedPlan {
@@ -111,6 +119,21 @@ struct MixedPlan {
   int span;                // bins of one core workgroup's row: cpw + 2 kmax + 1
   int ncwg;                // core workgroups per task
   double* core_part;       // [ntasks][ncwg][span] per-workgroup core sums
+  // Far field of region I (gx_far.hip): multipole moments of 64-line leaves and of their
+  // binary tree; blocks take the lines far from them from the moments instead of line
+  // by line.  far == 0: every line of a block is evaluated directly.
+  int far;
+  int far_bins;            // bins of one block (64 R)
+  int far_nlev;            // tree levels (level 0: leaves)
+  int far_off[kFarMaxLevels];   // first cluster of each level
+  int far_n[kFarMaxLevels];     // clusters of each level
+  int far_nclus;           // clusters of all levels
+  double4* far_geom;       // [far_nclus]: centre, lowest nu, highest nu (shared by the tasks)
+  double* far_rec;         // [ntasks][far_nclus][kFarRec]: radius, max 1/g, moments
+  int4* far_rng;           // [ntasks][nblocks]: far line ranges [x, y) and [z, w) of each block
+  int* far_list;           // [ntasks][nblocks][kFarList]: clusters of each block, level << 24 | index
+  int* far_cnt;            // [ntasks][nblocks]
+  unsigned long long* far_evals;   // line-bin pairs taken from moments
 };
 
 constexpr int kEvalSlots = 64;       // eval counters (spread to avoid one hot atomic)
@@ -142,6 +165,10 @@ hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t
 // R = bins per lane (4 or 8); a work item (one wave) covers mixed_bins_per_block(R) bins.
 // launch_accum_mixed: region I (and the segment reduction); launch_core_mixed: the W4 core pass.
 hipError_t launch_accum_mixed(const MixedPlan& plan, int R, hipStream_t s);
+// Far field (gx_far.hip): cluster geometry, moments and block plans (before region I),
+// and the far-field sums added to the grids.
+hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s);
+hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s);
 hipError_t launch_core_mixed(const MixedPlan& plan, hipStream_t s);
 int mixed_occupancy(int R);   // resident workgroups (kWavesPerGroup waves each) per CU of the mixed kernel
 // LDS bytes of a core-pass workgroup with the given row span (0: the row does not fit LDS)

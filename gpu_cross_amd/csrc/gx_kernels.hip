@@ -410,7 +410,7 @@ __device__ __forceinline__ void wave_lds_handover() {
 }
 
 template <int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ? 5 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ? 6 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
   constexpr int kBins = MixedShape<R>::kBins;
   constexpr int kWaves = kBlock / 64;
   constexpr int kVec = StagedRec<R>::kVec;
@@ -452,6 +452,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
     const int len = rg.y - rg.x;
     const int lo = rg.x + (int)(((long long)len * seg) / P.nseg);
     const int hi = rg.x + (int)(((long long)len * (seg + 1)) / P.nseg);
+    // lines whose sums this block takes from far-field moments (k_far_eval)
+    int4 fr = P.far ? P.far_rng[(size_t)task * P.nblocks + b] : make_int4(0, 0, 0, 0);
+    fr.x = __builtin_amdgcn_readfirstlane(fr.x);
+    fr.y = __builtin_amdgcn_readfirstlane(fr.y);
+    fr.z = __builtin_amdgcn_readfirstlane(fr.z);
+    fr.w = __builtin_amdgcn_readfirstlane(fr.w);
 
     float sum[R], part[R];
 #pragma unroll
@@ -475,10 +481,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
     };
 
     for (int base = lo; base < hi; base += 64) {
+      if (base >= fr.x && base < fr.y) base = fr.y;
+      if (base >= fr.z && base < fr.w) base = fr.w;
+      if (base >= hi) break;
       const int i = base + lane;
       bool fa = false, fx = false, ed = false;
       float4 rec[kVec];
-      if (i < hi) {
+      if (i < hi && !(i >= fr.x && i < fr.y) && !(i >= fr.z && i < fr.w)) {
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
         const double nu = v0 == 0.0 ? 1e-6 : v0;

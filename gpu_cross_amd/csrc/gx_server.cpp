@@ -143,6 +143,7 @@ size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& b
     const size_t nb = (M + 1023) / 1024 + 1;
     need += nb * B * 16 * 1024 * sizeof(float2) / 4;   // partials, upper estimate
     need += (M + 4096) * B * 4 * sizeof(double);       // core-pass rows, upper estimate
+    need += far_field_bytes((int)B, (long long)nlines, (int)(M / 64 + 1));
   } else {
     need += std::max(sl.der.cap, nlines) * 32;
   }

@@ -82,7 +82,8 @@ struct gx_table {
   hipGraphExec_t exec = nullptr;
   bool graph_ran = false;
   float eager_intensity_ms = 0, eager_accumulate_ms = 0;
-  float eager_region1_ms = 0, eager_core_ms = 0;
+  float eager_region1_ms = 0, eager_core_ms = 0, eager_far_ms = 0;
+  unsigned long long eager_far_evals = 0;
   unsigned long long* core_evals = nullptr;   // kEvalSlots counters of the core pass
   int R = 4, ips = 4;
 };
@@ -228,8 +229,11 @@ int autotune(gx_table* t) {
 }
 
 // Region-I and core-pass times of the last timed run, summed over the launches.
-int split_times(gx_table* t, float* region1, float* core) {
+int split_times(gx_table* t, float* region1, float* core, float* far = nullptr,
+                unsigned long long* far_evals = nullptr) {
   *region1 = *core = 0.0f;
+  if (far) *far = 0.0f;
+  if (far_evals) *far_evals = 0;
   if (!mixed_table(t)) return GX_OK;
   for (Launch& l : t->launches) {
     if (!l.ev[0] || l.run.plan.nitems <= 0) continue;
@@ -239,6 +243,14 @@ int split_times(gx_table* t, float* region1, float* core) {
       return hip_fail(e, "table: events");
     *region1 += a;
     *core += b;
+    if (far) {
+      float fm = 0;
+      unsigned long long fe = 0;
+      const int rc = l.run.far_time(&fm, &fe);
+      if (rc) return rc;
+      *far += fm;
+      if (far_evals) *far_evals += fe;
+    }
   }
   return GX_OK;
 }
@@ -397,8 +409,10 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
     size_t free_b = 0, total_b = 0;
     if ((e = hipMemGetInfo(&free_b, &total_b))) return bail(hip_fail(e, "hipMemGetInfo"));
     const size_t headroom = size_t(2) << 30;
-    // amp, y, 1/alpha, core range per line, and the core pass's rows (~3 doubles per bin)
-    const size_t per_point = 28 * std::max<size_t>(n, 1) + 32 * (size_t)t->M + (size_t(64) << 10);
+    // amp, y, 1/alpha, core range per line, the core pass's rows (~3 doubles per bin) and
+    // the far-field moments and block plans
+    const size_t per_point = 28 * std::max<size_t>(n, 1) + 32 * (size_t)t->M + (size_t(64) << 10) +
+                             far_field_bytes(1, (long long)std::max<size_t>(n, 1), t->M / 64 + 1);
     const size_t avail = free_b > headroom ? free_b - headroom : 0;
     t->batch = (int)std::min<size_t>((size_t)npts, avail / per_point);
     if (t->batch < 1) return bail(fail(GX_OUT_OF_MEMORY, "table_create: not enough free HBM for one point"));
@@ -445,7 +459,8 @@ int gx_table_run(gx_table* t) {
     if ((e = hipEventElapsedTime(&t->eager_intensity_ms, t->ev[0], t->ev[1])) ||
         (e = hipEventElapsedTime(&t->eager_accumulate_ms, t->ev[1], t->ev[2])))
       return hip_fail(e, "table_run: events");
-    if ((rc = split_times(t, &t->eager_region1_ms, &t->eager_core_ms))) return rc;
+    if ((rc = split_times(t, &t->eager_region1_ms, &t->eager_core_ms, &t->eager_far_ms, &t->eager_far_evals)))
+      return rc;
     if ((e = hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal))) return hip_fail(e, "begin capture");
     rc = enqueue(t, false);
     hipGraph_t g = nullptr;
@@ -492,13 +507,19 @@ int gx_table_wait(gx_table* t, gx_timing* timing) {
   timing->region1_ms = b;
   timing->core_ms = 0.0f;
   timing->core_evals = 0;
+  timing->far_ms = 0.0f;
+  timing->far_evals = 0;
   if (mixed_table(t)) {
     if (t->graph_ran) {
       timing->region1_ms = t->eager_region1_ms;
       timing->core_ms = t->eager_core_ms;
+      timing->far_ms = t->eager_far_ms;
+      timing->far_evals = (long long)t->eager_far_evals;
     } else {
-      int rc = split_times(t, &timing->region1_ms, &timing->core_ms);
+      unsigned long long fe = 0;
+      int rc = split_times(t, &timing->region1_ms, &timing->core_ms, &timing->far_ms, &fe);
       if (rc) return rc;
+      timing->far_evals = (long long)fe;
     }
     if (t->core_evals) {
       unsigned long long slots[kEvalSlots], ce = 0;

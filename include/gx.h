@@ -77,6 +77,11 @@ typedef struct gx_timing {
                                (FP64: accumulate_ms) */
     float core_ms;          /* mixed precision: the W4 core pass (regions II-IV); FP64: 0 */
     long long core_evals;   /* mixed precision: evaluations made by the core pass; FP64: 0 */
+    float far_ms;           /* mixed precision: the far-field kernels of region I (moments, block
+                               plans, far sums); 0 when off */
+    long long far_evals;    /* mixed precision: region-I line-bin pairs summed from moments
+                               instead of one by one (the region-I kernel made
+                               evals - core_evals - far_evals) */
 } gx_timing;
 
 /* BaseManager(ProfileType) + GpuManager ctor (GPUManager.cpp:5-17): bind

@@ -142,6 +142,34 @@ def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
     assert err <= MIXED_TOL
 
 
+@pytest.mark.parametrize("T,P", [(300.0, 1e-4), (1500.0, 0.1), (3000.0, 10.0), (1000.0, 100.0)])
+def test_voigt_mixed_far_field(voigt_lines, monkeypatch, T, P):
+    """Region I from far-field moments (the default) against the FP64 oracle, against the
+    same run with the far field off (GX_FAR=0), and engaged: a share of the pairs came
+    from moments."""
+    ll = voigt_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 5900.0) & (ll.nu < 6300.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(6000.0, 6200.0, 40000)
+    gam = L.gamma_at(P)
+    print(f"voigt mixed far field T={T} P={P} lines={L.n}", flush=True)
+    on, t_on = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                           cutoff=25.0)
+    monkeypatch.setenv("GX_FAR", "0")
+    off, t_off = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                             cutoff=25.0)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=25.0)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    e_on, e_off = rel_err(on, ref), rel_err(off, ref)
+    share = t_on.far_evals / max(t_on.evals, 1)
+    print(f"  far field: {share:.1%} of pairs, {t_on.far_ms:.2f} ms; max rel err on {e_on:.3e} off {e_off:.3e}",
+          flush=True)
+    assert e_on <= MIXED_TOL and e_off <= MIXED_TOL
+    assert t_off.far_evals == 0 and t_on.far_evals > 0
+    assert t_on.far_evals + t_on.core_evals < t_on.evals
+
+
 @pytest.mark.parametrize("T,cutoff", [(300.0, 0.3), (300.0, 1.2), (3000.0, 1.2), (3000.0, 0.3), (1000.0, 3.0)])
 def test_voigt_mixed_short_windows(voigt_lines, T, cutoff):
     # Windows of 120-1200 bins against 256-bin blocks: both window ends in one block, one end
