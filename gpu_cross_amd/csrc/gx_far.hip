@@ -45,54 +45,86 @@ __global__ void k_far_geom(MixedPlan P, double4* __restrict__ geom) {
   geom[k] = make_double4(0.5 * (lo + hi), lo, hi, 0.0);
 }
 
-// Leaf moments of every task: one thread per (leaf, task).
+// 64-bit DPP read of lane (lane - shift within the row), 0 where the source is out of the row.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_take_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes, in a fixed order (scan within rows of 16, then the row totals
+// by row_bcast 15 and 31); lane 63 holds the total.
+__device__ __forceinline__ double wave_sum_f64_lane63(double v) {
+  v += dpp_take_f64<0x111, 0xf>(v);
+  v += dpp_take_f64<0x112, 0xf>(v);
+  v += dpp_take_f64<0x114, 0xf>(v);
+  v += dpp_take_f64<0x118, 0xf>(v);
+  v += dpp_take_f64<0x142, 0xa>(v);
+  v += dpp_take_f64<0x143, 0xc>(v);
+  return v;
+}
+
+// Leaf moments of every task: one wave per (leaf, task), one line per lane, the lanes'
+// terms summed by a fixed DPP tree.
 __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
 #pragma clang fp contract(off)
-  const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int leaf = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (leaf >= P.far_n[0]) return;
   const MixedTask tk = P.tasks[blockIdx.y];
-  const int l0 = leaf * kFarLeaf, l1 = min(P.nlines, l0 + kFarLeaf);
+  const int l = leaf * kFarLeaf + lane;
+  const bool act = l < P.nlines;
   const double c = P.far_geom[leaf].x;
-  double r2 = 0.0, amax = 0.0;
-  for (int l = l0; l < l1; ++l) {
-    const double nu = nu_if(P.nu[l]);
-    const double al = 1.0 / tk.g[l];
-    const double s = kPoleShift * al, im = tk.y[l] * al;
-    const double dp = nu + s - c, dm = nu - s - c;
-    r2 = fmax(r2, fmax(dp * dp, dm * dm) + im * im);
-    amax = fmax(amax, al);
+  double nu = c, g = 1.0, y = 0.0;
+  float amp = 0.0f;
+  if (act) {
+    nu = nu_if(P.nu[l]);
+    g = tk.g[l];
+    y = tk.y[l];
+    amp = tk.amp[l];
+  }
+  const double al = act ? 1.0 / g : 0.0;
+  const double s = kPoleShift * al, im = y * al;
+  const double dp = nu + s - c, dm = nu - s - c;
+  double r2 = act ? fmax(dp * dp, dm * dm) + im * im : 0.0;
+  double amax = al;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    r2 = fmax(r2, __shfl_xor(r2, o));
+    amax = fmax(amax, __shfl_xor(amax, o));
   }
   double r = sqrt(r2);
   if (!(r > 0.0)) r = 1.0;
   const double ir = 1.0 / r;
+  const double kap = act ? kKappa * (double)amp * al : 0.0;   // amp = S g
+  const double wi = im * ir;
   double q[kFarP];
 #pragma unroll
   for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
-  for (int l = l0; l < l1; ++l) {
-    const double nu = nu_if(P.nu[l]);
-    const double g = tk.g[l];
-    const double al = 1.0 / g;
-    const double kap = kKappa * (double)tk.amp[l] * al;       // amp = S g
-    const double s = kPoleShift * al, im = tk.y[l] * al * ir;
 #pragma unroll
-    for (int pole = 0; pole < 2; ++pole) {
-      const double wr = (pole ? nu - s - c : nu + s - c) * ir;
-      double pr = wr, pi = im;                                 // w^1
+  for (int pole = 0; pole < 2; ++pole) {
+    const double wr = (pole ? dm : dp) * ir;
+    double pr = wr, pi = wi;                                   // w^1
 #pragma unroll
-      for (int k = 0; k < kFarP; ++k) {
-        q[k] += kap * pi;
-        const double nr = pr * wr - pi * im;
-        const double ni = pr * im + pi * wr;
-        pr = nr;
-        pi = ni;
-      }
+    for (int k = 0; k < kFarP; ++k) {
+      q[k] += kap * pi;
+      const double nr = pr * wr - pi * wi;
+      const double ni = pr * wi + pi * wr;
+      pr = nr;
+      pi = ni;
     }
   }
-  double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
-  rec[0] = r;
-  rec[1] = amax;
 #pragma unroll
-  for (int k = 0; k < kFarP; ++k) rec[2 + k] = q[k];
+  for (int k = 0; k < kFarP; ++k) q[k] = wave_sum_f64_lane63(q[k]);
+  if (lane == 63) {
+    double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
+    rec[0] = r;
+    rec[1] = amax;
+#pragma unroll
+    for (int k = 0; k < kFarP; ++k) rec[2 + k] = q[k];
+  }
 }
 
 // Parents of level lev from their two children: centre shift a = (c_child - c) / r and
@@ -291,7 +323,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   k_far_geom<<<(plan.far_nclus + 255) / 256, 256, 0, s>>>(plan, plan.far_geom);
   k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
-  k_far_leaf<<<dim3((plan.far_n[0] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
+  k_far_leaf<<<dim3((plan.far_n[0] + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
   k_far_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
