@@ -272,8 +272,9 @@ void MixedRunner::release() {
   hipFree(d_tasks); hipFree(partial); hipFree(counter); hipFree(starts); hipFree(core_part);
   d_tasks = nullptr; partial = nullptr; counter = nullptr; starts = nullptr; core_part = nullptr;
   task_cap = 0; partial_cap = 0; cus = 0; starts_cap = 0; core_cap = 0;
-  hipFree(far_geom); hipFree(far_rec); hipFree(far_rng); hipFree(far_list); hipFree(far_cnt); hipFree(far_evals);
-  far_geom = nullptr; far_rec = nullptr; far_rng = nullptr; far_list = nullptr; far_cnt = nullptr;
+  hipFree(far_geom); hipFree(far_rec); hipFree(far_rng); hipFree(edge_rng); hipFree(far_list); hipFree(far_cnt);
+  hipFree(far_evals);
+  far_geom = nullptr; far_rec = nullptr; far_rng = nullptr; edge_rng = nullptr; far_list = nullptr; far_cnt = nullptr;
   far_evals = nullptr; far_geom_cap = far_rec_cap = far_blk_cap = 0;
   for (auto& ev : far_ev) {
     if (ev) hipEventDestroy(ev);
@@ -451,13 +452,15 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
       }
       if (need_blk > far_blk_cap) {
         hipFree(far_rng);
+        hipFree(edge_rng);
         hipFree(far_list);
         hipFree(far_cnt);
         far_rng = nullptr;
+        edge_rng = nullptr;
         far_list = nullptr;
         far_cnt = nullptr;
         far_blk_cap = 0;
-        if ((e = hipMalloc(&far_rng, sizeof(int4) * need_blk)) ||
+        if ((e = hipMalloc(&far_rng, sizeof(int4) * need_blk)) || (e = hipMalloc(&edge_rng, sizeof(int4) * need_blk)) ||
             (e = hipMalloc(&far_list, sizeof(int) * need_blk * kFarList)) ||
             (e = hipMalloc(&far_cnt, sizeof(int) * need_blk)))
           return hip_fail(e, "hipMalloc(far plans)");
@@ -469,6 +472,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     plan.far_geom = far_geom;
     plan.far_rec = far_rec;
     plan.far_rng = far_rng;
+    plan.edge_rng = edge_rng;
     plan.far_list = far_list;
     plan.far_cnt = far_cnt;
     plan.far_evals = far_evals;
@@ -480,7 +484,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
 size_t far_field_bytes(int ntasks, long long nlines, int nblocks) {
   const long long clusters = 2 * ((nlines + kFarLeaf - 1) / kFarLeaf) + 64;
   return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * kFarRec * sizeof(double)) +
-         (size_t)ntasks * nblocks * (sizeof(int4) + sizeof(int) * (kFarList + 1));
+         (size_t)ntasks * nblocks * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1));
 }
 
 int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie,

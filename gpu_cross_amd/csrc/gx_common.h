@@ -92,6 +92,7 @@ struct MixedRunner {
   double* far_rec = nullptr;
   size_t far_rec_cap = 0;      // doubles
   int4* far_rng = nullptr;
+  int4* edge_rng = nullptr;
   int* far_list = nullptr;
   int* far_cnt = nullptr;
   size_t far_blk_cap = 0;      // (task, block) slots

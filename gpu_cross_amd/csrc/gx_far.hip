@@ -274,6 +274,60 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
   if (far_lines) atomicAdd(P.far_evals, far_lines * (unsigned long long)P.far_bins);
 }
 
+// Lines whose windows end inside a block (ib <= w0 < ie < w1, bins j < ie) and lines whose
+// windows start inside it (w0 < ib < w1 <= ie, bins j >= ib), taken from moments by
+// k_far_edges when every leaf they touch serves the whole block; else left to the
+// region-I kernel.  One thread per (task, block).
+__global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.nblocks) return;
+  const int task = blockIdx.y;
+  const size_t slot = (size_t)task * P.nblocks + b;
+  int4 er = make_int4(0, 0, 0, 0);
+  const GridView gv = P.gv;
+  const int w0 = gv.start_idx + b * P.far_bins;
+  const int w1 = w0 + P.far_bins;
+  const int2 rg = P.ranges[b];
+  if (w1 <= gv.start_idx + gv.points && rg.y > rg.x) {
+    const double f_lo = gv.freq[w0], f_hi = gv.freq[w1 - 1];
+    const int L1 = lower_bound_int(P.ie, rg.x, rg.y, w1);        // ie >= w1 from here
+    const int L2 = lower_bound_int(P.ib, rg.x, rg.y, w0 + 1);    // ib > w0 from here
+    const int Lx = lower_bound_int(P.ie, rg.x, rg.y, w0 + 1);    // ie > w0 from here
+    const int Ly = lower_bound_int(P.ib, rg.x, rg.y, w1);        // ib >= w1 from here
+    const int em0 = Lx, em1 = min(L1, L2);
+    const int ep0 = max(L1, L2), ep1 = Ly;
+    const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+    auto leaves_usable = [&](int a, int z) -> bool {
+      for (int leaf = a / kFarLeaf; leaf * kFarLeaf < z; ++leaf) {
+        const double4 gm = P.far_geom[leaf];
+        const double* rc = recs + (size_t)leaf * kFarRec;
+        double dist, edge;
+        if (gm.z < f_lo) {
+          dist = f_lo - gm.x;
+          edge = f_lo - gm.z;
+        } else if (gm.y > f_hi) {
+          dist = gm.x - f_hi;
+          edge = gm.y - f_hi;
+        } else {
+          return false;
+        }
+        if (!(rc[0] <= kFarEta * dist && edge >= 15.001 * rc[1])) return false;
+      }
+      return true;
+    };
+    if (em1 > em0 && leaves_usable(em0, em1)) {
+      er.x = em0;
+      er.y = em1;
+    }
+    if (ep1 > ep0 && leaves_usable(ep0, ep1)) {
+      er.z = ep0;
+      er.w = ep1;
+    }
+  }
+  P.edge_rng[slot] = er;
+}
+
 // Far-field sums of one (task, block) per wave, added to the task's grid.
 template <int R>
 __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
@@ -317,6 +371,110 @@ __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
   for (int q = 0; q < R; ++q) tk.out[w0 + lane + 64 * q] += acc[q];
 }
 
+// First index in [a, z) with v[index] > key (v non-decreasing there).
+__device__ __forceinline__ int upper_bound_int(const int* __restrict__ v, int a, int z, int key) {
+  while (a < z) {
+    const int m = a + (z - a) / 2;
+    if (v[m] <= key) a = m + 1;
+    else z = m;
+  }
+  return a;
+}
+
+// Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
+// kernel's: c y (u + 1) / (u^2 + 2 y^2), u = x^2 + y^2 - 1/2, scaled by amp = S g).
+__device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedTask& tk, int a, int z, double f) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int l = a; l < z; ++l) {
+    const double x = (nu_if(P.nu[l]) - f) * tk.g[l];
+    const double y = tk.y[l];
+    const double y2 = y * y;
+    const double u = x * x + y2 - 0.5;
+    s += (double)tk.amp[l] * 0.5641896 * y * (u + 1.0) / (u * u + 2.0 * y2);
+  }
+  return s;
+}
+
+// Region I of the lines [a, z) (every one contributing) at bin f: leaves of the range
+// from the largest aligned clusters that serve f, partial leaves line by line.
+__device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, int a, int z,
+                            double f) {
+#pragma clang fp contract(off)
+  if (z <= a) return 0.0;
+  const int A = (a + kFarLeaf - 1) / kFarLeaf, B = z / kFarLeaf;
+  if (A >= B) return region1_lines(P, tk, a, z, f);
+  double s = region1_lines(P, tk, a, A * kFarLeaf, f);
+  int i = A;
+  while (i < B) {
+    int lev = min(P.far_nlev - 1, i ? __builtin_ctz(i) : 31);
+    bool took = false;
+    for (; lev >= 0; --lev) {
+      if (i + (1 << lev) > B) continue;
+      const int k = P.far_off[lev] + (i >> lev);
+      const double4 gm = P.far_geom[k];
+      const double* rc = recs + (size_t)k * kFarRec;
+      const double d = f - gm.x;
+      const double edge = gm.z < f ? f - gm.z : (gm.y > f ? gm.y - f : -1.0);
+      if (!(rc[0] <= kFarEta * fabs(d) && edge >= 15.001 * rc[1])) continue;
+      const double id = 1.0 / d;
+      const double t = rc[0] * id;
+      double q = rc[2 + kFarP - 1];
+#pragma unroll
+      for (int m = kFarP - 2; m >= 0; --m) q = q * t + rc[2 + m];
+      s += q * t * id;
+      i += 1 << lev;
+      took = true;
+      break;
+    }
+    if (!took) {                 // a leaf that does not serve f: its lines one by one
+      s += region1_lines(P, tk, i * kFarLeaf, min((i + 1) * kFarLeaf, z), f);
+      ++i;
+    }
+  }
+  return s + region1_lines(P, tk, max(B * kFarLeaf, a), z, f);
+}
+
+// Window-end lines of one (task, block) per wave: bin j gets the lines ending inside the
+// block with ie > j and those starting inside it with ib <= j.
+template <int R>
+__global__ __launch_bounds__(256) void k_far_edges(MixedPlan P) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= P.nblocks) return;
+  const int task = blockIdx.y;
+  const size_t slot = (size_t)task * P.nblocks + b;
+  const int4 er = P.edge_rng[slot];
+  if (er.x >= er.y && er.z >= er.w) return;
+  const MixedTask tk = P.tasks[task];
+  const GridView gv = P.gv;
+  const int w0 = gv.start_idx + b * 64 * R;
+  const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  unsigned long long pairs = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int j = w0 + lane + 64 * q;
+    const double f = gv.freq[j];
+    double acc = 0.0;
+    if (er.x < er.y) {
+      const int s0 = upper_bound_int(P.ie, er.x, er.y, j);     // ie > j from here
+      acc += range_sum(P, tk, recs, s0, er.y, f);
+      pairs += (unsigned long long)(er.y - s0);
+    }
+    if (er.z < er.w) {
+      const int p1 = upper_bound_int(P.ib, er.z, er.w, j);     // ib <= j before here
+      acc += range_sum(P, tk, recs, er.z, p1, f);
+      pairs += (unsigned long long)(p1 - er.z);
+    }
+    tk.out[j] += acc;
+  }
+  // pairs of this wave, counted once
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+  if (lane == 0 && pairs) atomicAdd(P.far_evals, pairs);
+}
+
 }  // namespace
 
 hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
@@ -327,16 +485,20 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
   k_far_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
+  k_edge_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
   return hipGetLastError();
 }
 
 hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
-  if (R == 4)
+  if (R == 4) {
     k_far_eval<4><<<grid, 256, 0, s>>>(plan);
-  else
+    k_far_edges<4><<<grid, 256, 0, s>>>(plan);
+  } else {
     k_far_eval<8><<<grid, 256, 0, s>>>(plan);
+    k_far_edges<8><<<grid, 256, 0, s>>>(plan);
+  }
   return hipGetLastError();
 }
 

@@ -131,6 +131,8 @@ struct MixedPlan {
   double4* far_geom;       // [far_nclus]: centre, lowest nu, highest nu (shared by the tasks)
   double* far_rec;         // [ntasks][far_nclus][kFarRec]: radius, max 1/g, moments
   int4* far_rng;           // [ntasks][nblocks]: far line ranges [x, y) and [z, w) of each block
+  int4* edge_rng;          // [ntasks][nblocks]: window-end lines from moments: [x, y) end inside the
+                           // block (bins j < ie), [z, w) start inside it (bins j >= ib)
   int* far_list;           // [ntasks][nblocks][kFarList]: clusters of each block, level << 24 | index
   int* far_cnt;            // [ntasks][nblocks]
   unsigned long long* far_evals;   // line-bin pairs taken from moments

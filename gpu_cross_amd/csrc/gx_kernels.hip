@@ -452,12 +452,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
     const int len = rg.y - rg.x;
     const int lo = rg.x + (int)(((long long)len * seg) / P.nseg);
     const int hi = rg.x + (int)(((long long)len * (seg + 1)) / P.nseg);
-    // lines whose sums this block takes from far-field moments (k_far_eval)
+    // lines whose sums this block takes from far-field moments (k_far_eval, k_far_edges)
     int4 fr = P.far ? P.far_rng[(size_t)task * P.nblocks + b] : make_int4(0, 0, 0, 0);
+    int4 er = P.far ? P.edge_rng[(size_t)task * P.nblocks + b] : make_int4(0, 0, 0, 0);
     fr.x = __builtin_amdgcn_readfirstlane(fr.x);
     fr.y = __builtin_amdgcn_readfirstlane(fr.y);
     fr.z = __builtin_amdgcn_readfirstlane(fr.z);
     fr.w = __builtin_amdgcn_readfirstlane(fr.w);
+    er.x = __builtin_amdgcn_readfirstlane(er.x);
+    er.y = __builtin_amdgcn_readfirstlane(er.y);
+    er.z = __builtin_amdgcn_readfirstlane(er.z);
+    er.w = __builtin_amdgcn_readfirstlane(er.w);
 
     float sum[R], part[R];
 #pragma unroll
@@ -481,13 +486,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
     };
 
     for (int base = lo; base < hi; base += 64) {
+      if (base >= er.x && base < er.y) base = er.y;
       if (base >= fr.x && base < fr.y) base = fr.y;
       if (base >= fr.z && base < fr.w) base = fr.w;
+      if (base >= er.z && base < er.w) base = er.w;
       if (base >= hi) break;
       const int i = base + lane;
       bool fa = false, fx = false, ed = false;
       float4 rec[kVec];
-      if (i < hi && !(i >= fr.x && i < fr.y) && !(i >= fr.z && i < fr.w)) {
+      if (i < hi && !(i >= fr.x && i < fr.y) && !(i >= fr.z && i < fr.w) && !(i >= er.x && i < er.y) &&
+          !(i >= er.z && i < er.w)) {
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
         const double nu = v0 == 0.0 ? 1e-6 : v0;
