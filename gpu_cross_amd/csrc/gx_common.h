@@ -96,6 +96,8 @@ struct MixedRunner {
   int* far_list = nullptr;
   int* far_cnt = nullptr;
   size_t far_blk_cap = 0;      // (task, block) slots
+  int* far_kind = nullptr;
+  int far_kind_cap = 0;        // tasks
   unsigned long long* far_evals = nullptr;
   hipEvent_t far_ev[2] = {nullptr, nullptr};
   bool far_timed = false;      // the last launch recorded far_ev

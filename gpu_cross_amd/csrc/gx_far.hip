@@ -13,13 +13,26 @@
 // ~25 eta^24 = 1e-6 (eta = 1/2) relative to the cluster's sum, since each line's
 // Im (zeta - nu_c)^k is gamma times a polynomial of degree k - 1.
 //
+// When every line is broad (Lorentz widths >= 1 cm^-1), the radius of a real-centred
+// cluster is at least its largest gamma and no cluster serves bins within twice that
+// distance.  Such points use a complex centre c = nu_c + i gamma_c instead:
+//     sum over the cluster = Re[ -i (1/D) sum_{k=0..23} M_k (r/D)^k ],  D = f - c,
+//     M_k = sum_lines kappa ((zeta+ - c)^k + (zeta- - c)^k) / r^k,
+// where r bounds |zeta - c|.  A line's share is kappa gamma / |zeta - f|^2, and the
+// tail of its series is at most kappa rho^24 / ((1 - rho) |D|), rho = r / |D|, so the
+// relative truncation error is at most rho^24 (1 + rho)^2 |D| / ((1 - rho) gamma),
+// which the planner holds below 5e-7 with the cluster's lowest gamma.
+//
 // Clusters: leaves of 64 consecutive lines of the sorted list and their binary tree.
 // Leaf moments come from the lines (k_far_leaf), a parent's from its children's by
-// the real shift nu_c' - nu_c (k_far_up).  For each (point, bin block), k_far_plan picks
+// the shift of the centre (k_far_up).  For each (point, bin block), k_far_plan picks
 // clusters among the lines whose windows cover the whole block (a contiguous slice of
-// the sorted list), far enough that the series converges and that no line's W4 core
-// reaches the block (so every pair is a region-I pair of the FP64 path too); the
+// the sorted list), far enough that the series converges and such that no line's W4
+// core reaches the block (so every pair is a region-I pair of the FP64 path too); the
 // region-I kernel skips those lines and k_far_eval adds their sums in FP64.
+//
+// Cluster record (kFarRec doubles): radius, max 1/g, min y, lowest and highest gamma,
+// gamma_c (0 for real centres), then Q_1..Q_24 (real centres) or (Re, Im) of M_0..M_23.
 #include <hip/hip_runtime.h>
 
 #include "gx_internal.h"
@@ -66,6 +79,25 @@ __device__ __forceinline__ double wave_sum_f64_lane63(double v) {
   return v;
 }
 
+// Moment kind of each point: complex centres when the Lorentz widths of 4096 lines
+// spread over the list are all >= kFarCplxGamma.  The kind changes which clusters
+// serve which bins, never the bounds every served pair meets.
+__global__ __launch_bounds__(256) void k_far_kind(MixedPlan P) {
+  const MixedTask tk = P.tasks[blockIdx.x];
+  double gmin = 1e300;
+  for (int k = threadIdx.x; k < 4096; k += 256) {
+    const long long l = (long long)k * P.nlines / 4096;
+    gmin = fmin(gmin, tk.y[l] * (1.0 / tk.g[l]));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) gmin = fmin(gmin, __shfl_xor(gmin, o));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = gmin;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    P.far_kind[blockIdx.x] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3])) >= kFarCplxGamma ? 1 : 0;
+}
+
 // Leaf moments of every task: one wave per (leaf, task), one line per lane, the lanes'
 // terms summed by a fixed DPP tree.
 __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
@@ -74,6 +106,7 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   const int leaf = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (leaf >= P.far_n[0]) return;
   const MixedTask tk = P.tasks[blockIdx.y];
+  const bool cplx = P.far_kind[blockIdx.y] != 0;
   const int l = leaf * kFarLeaf + lane;
   const bool act = l < P.nlines;
   const double c = P.far_geom[leaf].x;
@@ -86,99 +119,177 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
     amp = tk.amp[l];
   }
   const double al = act ? 1.0 / g : 0.0;
-  const double s = kPoleShift * al, im = y * al;
+  const double s = kPoleShift * al, im = y * al;               // im: the line's gamma
   const double dp = nu + s - c, dm = nu - s - c;
-  double r2 = act ? fmax(dp * dp, dm * dm) + im * im : 0.0;
-  double amax = al;
+  double amax = al, ymin = act ? y : 1e300, glo = act ? im : 1e300, ghi = act ? im : -1e300;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    r2 = fmax(r2, __shfl_xor(r2, o));
     amax = fmax(amax, __shfl_xor(amax, o));
+    ymin = fmin(ymin, __shfl_xor(ymin, o));
+    glo = fmin(glo, __shfl_xor(glo, o));
+    ghi = fmax(ghi, __shfl_xor(ghi, o));
   }
+  const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
+  const double wim = im - gc;                                  // Im (zeta - centre)
+  double r2 = act ? fmax(dp * dp, dm * dm) + wim * wim : 0.0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
   double r = sqrt(r2);
   if (!(r > 0.0)) r = 1.0;
   const double ir = 1.0 / r;
   const double kap = act ? kKappa * (double)amp * al : 0.0;   // amp = S g
-  const double wi = im * ir;
-  double q[kFarP];
-#pragma unroll
-  for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
-#pragma unroll
-  for (int pole = 0; pole < 2; ++pole) {
-    const double wr = (pole ? dm : dp) * ir;
-    double pr = wr, pi = wi;                                   // w^1
-#pragma unroll
-    for (int k = 0; k < kFarP; ++k) {
-      q[k] += kap * pi;
-      const double nr = pr * wr - pi * wi;
-      const double ni = pr * wi + pi * wr;
-      pr = nr;
-      pi = ni;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kFarP; ++k) q[k] = wave_sum_f64_lane63(q[k]);
+  const double wi = wim * ir;
+  double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
   if (lane == 63) {
-    double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
     rec[0] = r;
     rec[1] = amax;
+    rec[2] = ymin;
+    rec[3] = glo;
+    rec[4] = ghi;
+    rec[5] = gc;
+  }
+  // powers of w for both poles, each moment summed over the wave as soon as it is formed
+  const double wrp = dp * ir, wrm = dm * ir;
+  if (!cplx) {
+    double pr = wrp, pi = wi, mr = wrm, mi = wi;               // w+^1, w-^1
 #pragma unroll
-    for (int k = 0; k < kFarP; ++k) rec[2 + k] = q[k];
+    for (int k = 0; k < kFarP; ++k) {
+      const double q = wave_sum_f64_lane63(kap * pi + kap * mi);
+      if (lane == 63) rec[kFarHead + k] = q;
+      const double nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
+      const double n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
+      pr = nr;
+      pi = ni;
+      mr = n2r;
+      mi = n2i;
+    }
+  } else {
+    double pr = 1.0, pi = 0.0, mr = 1.0, mi = 0.0;             // w+^0, w-^0
+#pragma unroll
+    for (int k = 0; k < kFarP; ++k) {
+      const double qr = wave_sum_f64_lane63(kap * pr + kap * mr);
+      const double qi = wave_sum_f64_lane63(kap * pi + kap * mi);
+      if (lane == 63) {
+        rec[kFarHead + 2 * k] = qr;
+        rec[kFarHead + 2 * k + 1] = qi;
+      }
+      const double nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
+      const double n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
+      pr = nr;
+      pi = ni;
+      mr = n2r;
+      mi = n2i;
+    }
   }
 }
 
-// Parents of level lev from their two children: centre shift a = (c_child - c) / r and
-// scale b = r_child / r, Q_k = sum_m C(k, m) a^(k-m) b^m Q_m(child).
+// Parents of level lev from their two children.  Real centres: shift a = (c_child - c) / r
+// and scale b = r_child / r, Q_k = sum_m C(k, m) a^(k-m) b^m Q_m(child).  Complex centres:
+// the same with a complex shift, M_k = sum_m C(k, m) a^(k-m) b^m M_m(child), formed by
+// repeated Taylor-shift passes v_k += a v_(k-1) over v_m = b^m M_m(child).
 __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.far_n[lev]) return;
   const size_t tbase = (size_t)blockIdx.y * P.far_nclus;
+  const bool cplx = P.far_kind[blockIdx.y] != 0;
   const int self = P.far_off[lev] + i;
   const double c = P.far_geom[self].x;
   const int nch = min(2, P.far_n[lev - 1] - 2 * i);
-  double r = 0.0, amax = 0.0;
+  double amax = 0.0, ymin = 1e300, glo = 1e300, ghi = -1e300;
+  for (int h = 0; h < nch; ++h) {
+    const double* cr = P.far_rec + (tbase + P.far_off[lev - 1] + 2 * i + h) * kFarRec;
+    amax = fmax(amax, cr[1]);
+    ymin = fmin(ymin, cr[2]);
+    glo = fmin(glo, cr[3]);
+    ghi = fmax(ghi, cr[4]);
+  }
+  const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
+  double r = 0.0;
   for (int h = 0; h < nch; ++h) {
     const int ch = P.far_off[lev - 1] + 2 * i + h;
     const double* cr = P.far_rec + (tbase + ch) * kFarRec;
-    r = fmax(r, fabs(P.far_geom[ch].x - c) + cr[0]);
-    amax = fmax(amax, cr[1]);
+    const double dx = P.far_geom[ch].x - c, dy = cr[5] - gc;
+    r = fmax(r, (cplx ? sqrt(dx * dx + dy * dy) : fabs(dx)) + cr[0]);
   }
   if (!(r > 0.0)) r = 1.0;
-  double q[kFarP];
-#pragma unroll
-  for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
-  for (int h = 0; h < nch; ++h) {
-    const int ch = P.far_off[lev - 1] + 2 * i + h;
-    const double* cr = P.far_rec + (tbase + ch) * kFarRec;
-    const double a = (P.far_geom[ch].x - c) / r, b = cr[0] / r;
-    double pa[kFarP], qb[kFarP];                               // a^j, b^m Q_m
-    pa[0] = 1.0;
-    double bm = b;
-#pragma unroll
-    for (int j = 0; j < kFarP; ++j) {
-      if (j) pa[j] = pa[j - 1] * a;
-      qb[j] = bm * cr[2 + j];
-      bm *= b;
-    }
-    // Q_k (k = 1..P) += sum_{m=1..k} C(k, m) a^(k-m) b^m Q_m
-#pragma unroll
-    for (int k = 1; k <= kFarP; ++k) {
-      double cb = k;                                           // C(k, 1)
-      double acc = 0.0;
-#pragma unroll
-      for (int m = 1; m <= k; ++m) {
-        acc += cb * pa[k - m] * qb[m - 1];
-        cb = cb * (double)(k - m) / (double)(m + 1);
-      }
-      q[k - 1] += acc;
-    }
-  }
   double* rec = P.far_rec + (tbase + self) * kFarRec;
   rec[0] = r;
   rec[1] = amax;
+  rec[2] = ymin;
+  rec[3] = glo;
+  rec[4] = ghi;
+  rec[5] = gc;
+  if (!cplx) {
+    double q[kFarP];
 #pragma unroll
-  for (int k = 0; k < kFarP; ++k) rec[2 + k] = q[k];
+    for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
+    for (int h = 0; h < nch; ++h) {
+      const int ch = P.far_off[lev - 1] + 2 * i + h;
+      const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+      const double a = (P.far_geom[ch].x - c) / r, b = cr[0] / r;
+      double pa[kFarP], qb[kFarP];                             // a^j, b^m Q_m
+      pa[0] = 1.0;
+      double bm = b;
+#pragma unroll
+      for (int j = 0; j < kFarP; ++j) {
+        if (j) pa[j] = pa[j - 1] * a;
+        qb[j] = bm * cr[kFarHead + j];
+        bm *= b;
+      }
+      // Q_k (k = 1..P) += sum_{m=1..k} C(k, m) a^(k-m) b^m Q_m
+#pragma unroll
+      for (int k = 1; k <= kFarP; ++k) {
+        double cb = k;                                         // C(k, 1)
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 1; m <= k; ++m) {
+          acc += cb * pa[k - m] * qb[m - 1];
+          cb = cb * (double)(k - m) / (double)(m + 1);
+        }
+        q[k - 1] += acc;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kFarP; ++k) rec[kFarHead + k] = q[k];
+  } else {
+    double qr[kFarP], qi[kFarP];
+#pragma unroll
+    for (int k = 0; k < kFarP; ++k) qr[k] = qi[k] = 0.0;
+    for (int h = 0; h < nch; ++h) {
+      const int ch = P.far_off[lev - 1] + 2 * i + h;
+      const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+      const double ar = (P.far_geom[ch].x - c) / r, ai = (cr[5] - gc) / r, b = cr[0] / r;
+      double vr[kFarP], vi[kFarP];
+      double bm = 1.0;
+#pragma unroll
+      for (int m = 0; m < kFarP; ++m) {
+        vr[m] = bm * cr[kFarHead + 2 * m];
+        vi[m] = bm * cr[kFarHead + 2 * m + 1];
+        bm *= b;
+      }
+#pragma unroll
+      for (int j = 1; j < kFarP; ++j) {
+#pragma unroll
+        for (int k = kFarP - 1; k >= j; --k) {
+          const double nr = vr[k] + (ar * vr[k - 1] - ai * vi[k - 1]);
+          const double ni = vi[k] + (ar * vi[k - 1] + ai * vr[k - 1]);
+          vr[k] = nr;
+          vi[k] = ni;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kFarP; ++k) {
+        qr[k] += vr[k];
+        qi[k] += vi[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kFarP; ++k) {
+      rec[kFarHead + 2 * k] = qr[k];
+      rec[kFarHead + 2 * k + 1] = qi[k];
+    }
+  }
 }
 
 __global__ void k_far_reset(unsigned long long* c) { *c = 0ull; }
@@ -191,6 +302,49 @@ __device__ __forceinline__ int lower_bound_int(const int* __restrict__ v, int a,
     else z = m;
   }
   return a;
+}
+
+// Whether cluster (geometry gm, record rc) serves every bin of [f_lo, f_hi]: no line of it
+// has its W4 core there (its lowest y is >= 15, or its lines are 15 Doppler widths away)
+// and its series converges there within the truncation bound of its kind.
+__device__ __forceinline__ bool serves(const double4 gm, const double* __restrict__ rc, bool cplx, double f_lo,
+                                       double f_hi) {
+#pragma clang fp contract(off)
+  const double dist = fmax(fmax(f_lo - gm.x, gm.x - f_hi), 0.0);      // centre to the bins
+  const double edge = fmax(fmax(f_lo - gm.z, gm.y - f_hi), 0.0);      // the lines' nu range to the bins
+  if (!(rc[2] >= 15.0 || edge >= 15.001 * rc[1])) return false;
+  if (!cplx) return rc[0] <= kFarEta * dist;
+  const double D = sqrt(dist * dist + rc[5] * rc[5]);
+  const double rho = rc[0] / D;
+  if (!(rho <= kFarEta)) return false;
+  const double r2 = rho * rho, r4 = r2 * r2, r8 = r4 * r4;
+  return r8 * r8 * r8 * (1.0 + rho) * (1.0 + rho) * D <= kFarTol * (1.0 - rho) * rc[3];
+}
+
+// Sum of a cluster (record rc, centre nu_c) at bin f.
+__device__ __forceinline__ double cluster_sum(const double* __restrict__ rc, double nu_c, bool cplx, double f) {
+#pragma clang fp contract(off)
+  if (!cplx) {
+    const double id = 1.0 / (f - nu_c);
+    const double t = rc[0] * id;
+    double q = rc[kFarHead + kFarP - 1];
+#pragma unroll
+    for (int m = kFarP - 2; m >= 0; --m) q = q * t + rc[kFarHead + m];
+    return q * t * id;
+  }
+  const double dr = f - nu_c, di = -rc[5];                    // D = f - c
+  const double inv = 1.0 / (dr * dr + di * di);
+  const double ir = dr * inv, ii = -di * inv;                 // 1/D
+  const double tr = rc[0] * ir, ti = rc[0] * ii;              // r/D
+  double sr = rc[kFarHead + 2 * (kFarP - 1)], si = rc[kFarHead + 2 * (kFarP - 1) + 1];
+#pragma unroll
+  for (int m = kFarP - 2; m >= 0; --m) {
+    const double nr = (sr * tr - si * ti) + rc[kFarHead + 2 * m];
+    const double ni = (sr * ti + si * tr) + rc[kFarHead + 2 * m + 1];
+    sr = nr;
+    si = ni;
+  }
+  return sr * ii + si * ir;                                   // Re(-i s / D) = Im(s / D)
 }
 
 // The far clusters of one (task, block): one thread each.
@@ -215,23 +369,10 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
     const int leafL = (L1 + kFarLeaf - 1) / kFarLeaf, leafR = L2 / kFarLeaf;
     const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
     int* list = P.far_list + slot * kFarList;
-    // a cluster serves the block when its poles are within eta of the distance to the
-    // block and its lines are at least 15 Doppler widths from every bin (no W4 core)
+    const bool cplx = P.far_kind[task] != 0;
     auto usable = [&](int lev, int idx) -> bool {
       const int k = P.far_off[lev] + idx;
-      const double4 gm = P.far_geom[k];
-      const double* rc = recs + (size_t)k * kFarRec;
-      double dist, edge;
-      if (gm.z < f_lo) {
-        dist = f_lo - gm.x;
-        edge = f_lo - gm.z;
-      } else if (gm.y > f_hi) {
-        dist = gm.x - f_hi;
-        edge = gm.y - f_hi;
-      } else {
-        return false;
-      }
-      return rc[0] <= kFarEta * dist && edge >= 15.001 * rc[1];
+      return serves(P.far_geom[k], recs + (size_t)k * kFarRec, cplx, f_lo, f_hi);
     };
     int i = leafL;
     if (leafL < leafR) {
@@ -298,22 +439,10 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
     const int em0 = Lx, em1 = min(L1, L2);
     const int ep0 = max(L1, L2), ep1 = Ly;
     const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+    const bool cplx = P.far_kind[task] != 0;
     auto leaves_usable = [&](int a, int z) -> bool {
-      for (int leaf = a / kFarLeaf; leaf * kFarLeaf < z; ++leaf) {
-        const double4 gm = P.far_geom[leaf];
-        const double* rc = recs + (size_t)leaf * kFarRec;
-        double dist, edge;
-        if (gm.z < f_lo) {
-          dist = f_lo - gm.x;
-          edge = f_lo - gm.z;
-        } else if (gm.y > f_hi) {
-          dist = gm.x - f_hi;
-          edge = gm.y - f_hi;
-        } else {
-          return false;
-        }
-        if (!(rc[0] <= kFarEta * dist && edge >= 15.001 * rc[1])) return false;
-      }
+      for (int leaf = a / kFarLeaf; leaf * kFarLeaf < z; ++leaf)
+        if (!serves(P.far_geom[leaf], recs + (size_t)leaf * kFarRec, cplx, f_lo, f_hi)) return false;
       return true;
     };
     if (em1 > em0 && leaves_usable(em0, em1)) {
@@ -350,21 +479,19 @@ __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
     f[q] = gv.freq[w0 + lane + 64 * q];       // far clusters are planned for whole blocks only
     acc[q] = 0.0;
   }
+  const bool cplx = P.far_kind[task] != 0;
   for (int e = 0; e < cnt; ++e) {
     const int ent = list[e];
     const int lev = ent >> 24, idx = ent & 0xffffff;
     const int k = P.far_off[lev] + idx;
     const double c = P.far_geom[k].x;
     const double* rc = recs + (size_t)k * kFarRec;
-    const double r = rc[0];
+    if (cplx) {
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const double id = 1.0 / (f[q] - c);
-      const double t = r * id;
-      double s = rc[2 + kFarP - 1];
+      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, c, true, f[q]);
+    } else {
 #pragma unroll
-      for (int m = kFarP - 2; m >= 0; --m) s = s * t + rc[2 + m];
-      acc[q] += s * t * id;
+      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, c, false, f[q]);
     }
   }
 #pragma unroll
@@ -398,8 +525,8 @@ __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedT
 
 // Region I of the lines [a, z) (every one contributing) at bin f: leaves of the range
 // from the largest aligned clusters that serve f, partial leaves line by line.
-__device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, int a, int z,
-                            double f) {
+__device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, bool cplx,
+                            int a, int z, double f) {
 #pragma clang fp contract(off)
   if (z <= a) return 0.0;
   const int A = (a + kFarLeaf - 1) / kFarLeaf, B = z / kFarLeaf;
@@ -414,15 +541,8 @@ __device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const doubl
       const int k = P.far_off[lev] + (i >> lev);
       const double4 gm = P.far_geom[k];
       const double* rc = recs + (size_t)k * kFarRec;
-      const double d = f - gm.x;
-      const double edge = gm.z < f ? f - gm.z : (gm.y > f ? gm.y - f : -1.0);
-      if (!(rc[0] <= kFarEta * fabs(d) && edge >= 15.001 * rc[1])) continue;
-      const double id = 1.0 / d;
-      const double t = rc[0] * id;
-      double q = rc[2 + kFarP - 1];
-#pragma unroll
-      for (int m = kFarP - 2; m >= 0; --m) q = q * t + rc[2 + m];
-      s += q * t * id;
+      if (!serves(gm, rc, cplx, f, f)) continue;
+      s += cluster_sum(rc, gm.x, cplx, f);
       i += 1 << lev;
       took = true;
       break;
@@ -451,6 +571,7 @@ __global__ __launch_bounds__(256) void k_far_edges(MixedPlan P) {
   const GridView gv = P.gv;
   const int w0 = gv.start_idx + b * 64 * R;
   const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  const bool cplx = P.far_kind[task] != 0;
   unsigned long long pairs = 0;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -459,12 +580,12 @@ __global__ __launch_bounds__(256) void k_far_edges(MixedPlan P) {
     double acc = 0.0;
     if (er.x < er.y) {
       const int s0 = upper_bound_int(P.ie, er.x, er.y, j);     // ie > j from here
-      acc += range_sum(P, tk, recs, s0, er.y, f);
+      acc += range_sum(P, tk, recs, cplx, s0, er.y, f);
       pairs += (unsigned long long)(er.y - s0);
     }
     if (er.z < er.w) {
       const int p1 = upper_bound_int(P.ib, er.z, er.w, j);     // ib <= j before here
-      acc += range_sum(P, tk, recs, er.z, p1, f);
+      acc += range_sum(P, tk, recs, cplx, er.z, p1, f);
       pairs += (unsigned long long)(p1 - er.z);
     }
     tk.out[j] += acc;
@@ -481,6 +602,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   k_far_geom<<<(plan.far_nclus + 255) / 256, 256, 0, s>>>(plan, plan.far_geom);
   k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
+  k_far_kind<<<plan.ntasks, 256, 0, s>>>(plan);
   k_far_leaf<<<dim3((plan.far_n[0] + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);

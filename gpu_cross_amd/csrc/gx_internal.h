@@ -85,10 +85,14 @@ struct MixedTask {
 };
 
 // Far field of region I: moments per cluster, leaf size, list length, levels.
-constexpr int kFarP = 24;            // moments Q_1..Q_24 (truncation ~ 25 eta^24 relative)
+constexpr int kFarP = 24;            // moments per cluster (truncation ~ 25 eta^24 relative, real centres)
 constexpr double kFarEta = 0.5;      // a cluster of radius r serves bins at distance >= r / eta
 constexpr int kFarLeaf = 64;         // lines per leaf
-constexpr int kFarRec = 2 + kFarP;   // radius, max 1/g, moments
+constexpr int kFarHead = 6;          // radius, max 1/g, min y, lowest and highest gamma, gamma of the centre
+constexpr int kFarRec = kFarHead + 2 * kFarP;   // head, then P real or P complex moments
+constexpr double kFarTol = 5e-7;     // bound on the relative truncation error of complex-centred sums
+constexpr double kFarCplxGamma = 1.0;   // points whose sampled Lorentz widths are all >= this (cm^-1)
+                                        // take complex-centred moments
 constexpr int kFarList = 128;        // clusters per block at most
 constexpr int kFarMaxLevels = 32;
 
@@ -129,7 +133,8 @@ struct MixedPlan {
   int far_n[kFarMaxLevels];     // clusters of each level
   int far_nclus;           // clusters of all levels
   double4* far_geom;       // [far_nclus]: centre, lowest nu, highest nu (shared by the tasks)
-  double* far_rec;         // [ntasks][far_nclus][kFarRec]: radius, max 1/g, moments
+  double* far_rec;         // [ntasks][far_nclus][kFarRec]: head and moments of each cluster
+  int* far_kind;           // [ntasks]: 0 real-centred moments, 1 complex-centred
   int4* far_rng;           // [ntasks][nblocks]: far line ranges [x, y) and [z, w) of each block
   int4* edge_rng;          // [ntasks][nblocks]: window-end lines from moments: [x, y) end inside the
                            // block (bins j < ie), [z, w) start inside it (bins j >= ib)

@@ -142,11 +142,13 @@ def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
     assert err <= MIXED_TOL
 
 
-@pytest.mark.parametrize("T,P", [(300.0, 1e-4), (1500.0, 0.1), (3000.0, 10.0), (1000.0, 100.0)])
+@pytest.mark.parametrize("T,P", [(300.0, 1e-4), (1500.0, 0.1), (3000.0, 10.0), (1000.0, 100.0), (300.0, 100.0),
+                                 (3000.0, 100.0)])
 def test_voigt_mixed_far_field(voigt_lines, monkeypatch, T, P):
     """Region I from far-field moments (the default) against the FP64 oracle, against the
     same run with the far field off (GX_FAR=0), and engaged: a share of the pairs came
-    from moments."""
+    from moments.  At 100 atm every Lorentz width is >= 1 cm^-1 and the moments are
+    complex-centred."""
     ll = voigt_lines
     Q = po.partition(ll.state_e, ll.state_g, T)
     sel = np.nonzero((ll.nu > 5900.0) & (ll.nu < 6300.0))[0]
