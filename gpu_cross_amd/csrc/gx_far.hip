@@ -58,27 +58,6 @@ __global__ void k_far_geom(MixedPlan P, double4* __restrict__ geom) {
   geom[k] = make_double4(0.5 * (lo + hi), lo, hi, 0.0);
 }
 
-// 64-bit DPP read of lane (lane - shift within the row), 0 where the source is out of the row.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_take_f64(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// Sum over the 64 lanes, in a fixed order (scan within rows of 16, then the row totals
-// by row_bcast 15 and 31); lane 63 holds the total.
-__device__ __forceinline__ double wave_sum_f64_lane63(double v) {
-  v += dpp_take_f64<0x111, 0xf>(v);
-  v += dpp_take_f64<0x112, 0xf>(v);
-  v += dpp_take_f64<0x114, 0xf>(v);
-  v += dpp_take_f64<0x118, 0xf>(v);
-  v += dpp_take_f64<0x142, 0xa>(v);
-  v += dpp_take_f64<0x143, 0xc>(v);
-  return v;
-}
-
 // Moment kind of each point: complex centres when the Lorentz widths of 4096 lines
 // spread over the list are all >= kFarCplxGamma.  The kind changes which clusters
 // serve which bins, never the bounds every served pair meets.
@@ -98,8 +77,33 @@ __global__ __launch_bounds__(256) void k_far_kind(MixedPlan P) {
     P.far_kind[blockIdx.x] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3])) >= kFarCplxGamma ? 1 : 0;
 }
 
+// Sums over the 64 lanes of N values per lane (N a multiple of 8), by halving: at lane
+// distance 32, 16 and 8 each lane keeps half of its values and adds the partner's copy of
+// that half, then the N/8 values left are summed over the 8 lanes (distances 4, 2, 1).
+// Afterwards every lane holds the totals of values first(lane) + [0, N/8), first(lane) =
+// N/2 bit5 + N/4 bit4 + N/8 bit3 of its lane index.  Fixed order (a + b = b + a in IEEE):
+// the totals do not depend on scheduling.
+template <int N>
+__device__ __forceinline__ void wave_sums_f64(double (&v)[N], int lane) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int d = 32, h = N / 2; d >= 8; d >>= 1, h >>= 1) {
+    const bool up = (lane & d) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double keep = up ? v[h + i] : v[i];
+      const double send = up ? v[i] : v[h + i];
+      v[i] = keep + __shfl_xor(send, d);
+    }
+  }
+#pragma unroll
+  for (int d = 4; d >= 1; d >>= 1)
+#pragma unroll
+    for (int i = 0; i < N / 8; ++i) v[i] += __shfl_xor(v[i], d);
+}
+
 // Leaf moments of every task: one wave per (leaf, task), one line per lane, the lanes'
-// terms summed by a fixed DPP tree.
+// terms summed by wave_sums_f64.
 __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   const double kap = act ? kKappa * (double)amp * al : 0.0;   // amp = S g
   const double wi = wim * ir;
   double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
-  if (lane == 63) {
+  if (lane == 0) {
     rec[0] = r;
     rec[1] = amax;
     rec[2] = ymin;
@@ -148,14 +152,16 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
     rec[4] = ghi;
     rec[5] = gc;
   }
-  // powers of w for both poles, each moment summed over the wave as soon as it is formed
+  // every lane's terms for all moments, then one halving reduction over the wave; lanes
+  // 8m (m < 8) write the totals they hold
   const double wrp = dp * ir, wrm = dm * ir;
+  const int first = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
   if (!cplx) {
+    double v[kFarP];
     double pr = wrp, pi = wi, mr = wrm, mi = wi;               // w+^1, w-^1
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) {
-      const double q = wave_sum_f64_lane63(kap * pi + kap * mi);
-      if (lane == 63) rec[kFarHead + k] = q;
+      v[k] = kap * pi + kap * mi;
       const double nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
       const double n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
       pr = nr;
@@ -163,22 +169,29 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
       mr = n2r;
       mi = n2i;
     }
+    wave_sums_f64(v, lane);
+    if ((lane & 7) == 0) {
+#pragma unroll
+      for (int t = 0; t < kFarP / 8; ++t) rec[kFarHead + first * (kFarP / 8) + t] = v[t];
+    }
   } else {
+    double v[2 * kFarP];                                       // (Re, Im) of M_0..M_(P-1)
     double pr = 1.0, pi = 0.0, mr = 1.0, mi = 0.0;             // w+^0, w-^0
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) {
-      const double qr = wave_sum_f64_lane63(kap * pr + kap * mr);
-      const double qi = wave_sum_f64_lane63(kap * pi + kap * mi);
-      if (lane == 63) {
-        rec[kFarHead + 2 * k] = qr;
-        rec[kFarHead + 2 * k + 1] = qi;
-      }
+      v[2 * k] = kap * pr + kap * mr;
+      v[2 * k + 1] = kap * pi + kap * mi;
       const double nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
       const double n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
       pr = nr;
       pi = ni;
       mr = n2r;
       mi = n2i;
+    }
+    wave_sums_f64(v, lane);
+    if ((lane & 7) == 0) {
+#pragma unroll
+      for (int t = 0; t < kFarP / 4; ++t) rec[kFarHead + first * (kFarP / 4) + t] = v[t];
     }
   }
 }
@@ -418,7 +431,10 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
 // Lines whose windows end inside a block (ib <= w0 < ie < w1, bins j < ie) and lines whose
 // windows start inside it (w0 < ib < w1 <= ie, bins j >= ib), taken from moments by
 // k_far_edges when every leaf they touch serves the whole block; else left to the
-// region-I kernel.  One thread per (task, block).
+// region-I kernel.  The end of each range that is the same for every bin of the block (the
+// last line ending inside it, the first line starting inside it) is moved to a leaf
+// boundary, the lines beyond it staying with the region-I kernel, so k_far_edges evaluates
+// at most one partial leaf per range and bin.  One thread per (task, block).
 __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
 #pragma clang fp contract(off)
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,8 +452,8 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
     const int L2 = lower_bound_int(P.ib, rg.x, rg.y, w0 + 1);    // ib > w0 from here
     const int Lx = lower_bound_int(P.ie, rg.x, rg.y, w0 + 1);    // ie > w0 from here
     const int Ly = lower_bound_int(P.ib, rg.x, rg.y, w1);        // ib >= w1 from here
-    const int em0 = Lx, em1 = min(L1, L2);
-    const int ep0 = max(L1, L2), ep1 = Ly;
+    const int em0 = Lx, em1 = min(L1, L2) / kFarLeaf * kFarLeaf;
+    const int ep0 = (max(L1, L2) + kFarLeaf - 1) / kFarLeaf * kFarLeaf, ep1 = Ly;
     const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
     const bool cplx = P.far_kind[task] != 0;
     auto leaves_usable = [&](int a, int z) -> bool {
@@ -555,41 +571,39 @@ __device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const doubl
   return s + region1_lines(P, tk, max(B * kFarLeaf, a), z, f);
 }
 
-// Window-end lines of one (task, block) per wave: bin j gets the lines ending inside the
-// block with ie > j and those starting inside it with ib <= j.
+// Window-end lines, one bin per lane (a block is 64 R bins, so every wave lies in one
+// block): bin j gets the lines ending inside its block with ie > j and those starting
+// inside it with ib <= j.
 template <int R>
 __global__ __launch_bounds__(256) void k_far_edges(MixedPlan P) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
-  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = blockIdx.x * 256 + threadIdx.x;                // bin of the chunk
+  const int b = __builtin_amdgcn_readfirstlane(t / (64 * R));
   if (b >= P.nblocks) return;
   const int task = blockIdx.y;
   const size_t slot = (size_t)task * P.nblocks + b;
   const int4 er = P.edge_rng[slot];
-  if (er.x >= er.y && er.z >= er.w) return;
+  if (er.x >= er.y && er.z >= er.w) return;                    // set for whole blocks only
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
-  const int w0 = gv.start_idx + b * 64 * R;
   const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const bool cplx = P.far_kind[task] != 0;
   unsigned long long pairs = 0;
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int j = w0 + lane + 64 * q;
-    const double f = gv.freq[j];
-    double acc = 0.0;
-    if (er.x < er.y) {
-      const int s0 = upper_bound_int(P.ie, er.x, er.y, j);     // ie > j from here
-      acc += range_sum(P, tk, recs, cplx, s0, er.y, f);
-      pairs += (unsigned long long)(er.y - s0);
-    }
-    if (er.z < er.w) {
-      const int p1 = upper_bound_int(P.ib, er.z, er.w, j);     // ib <= j before here
-      acc += range_sum(P, tk, recs, cplx, er.z, p1, f);
-      pairs += (unsigned long long)(p1 - er.z);
-    }
-    tk.out[j] += acc;
+  const int j = gv.start_idx + t;
+  const double f = gv.freq[j];
+  double acc = 0.0;
+  if (er.x < er.y) {
+    const int s0 = upper_bound_int(P.ie, er.x, er.y, j);       // ie > j from here
+    acc += range_sum(P, tk, recs, cplx, s0, er.y, f);
+    pairs += (unsigned long long)(er.y - s0);
   }
+  if (er.z < er.w) {
+    const int p1 = upper_bound_int(P.ib, er.z, er.w, j);       // ib <= j before here
+    acc += range_sum(P, tk, recs, cplx, er.z, p1, f);
+    pairs += (unsigned long long)(p1 - er.z);
+  }
+  tk.out[j] += acc;
   // pairs of this wave, counted once
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
@@ -614,12 +628,13 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
 hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
+  const dim3 grid_bins((unsigned)(((long long)plan.nblocks * 64 * R + 255) / 256), plan.ntasks);
   if (R == 4) {
     k_far_eval<4><<<grid, 256, 0, s>>>(plan);
-    k_far_edges<4><<<grid, 256, 0, s>>>(plan);
+    k_far_edges<4><<<grid_bins, 256, 0, s>>>(plan);
   } else {
     k_far_eval<8><<<grid, 256, 0, s>>>(plan);
-    k_far_edges<8><<<grid, 256, 0, s>>>(plan);
+    k_far_edges<8><<<grid_bins, 256, 0, s>>>(plan);
   }
   return hipGetLastError();
 }
