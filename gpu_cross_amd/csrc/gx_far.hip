@@ -9,13 +9,15 @@
 // every pole within r of it, and a bin at d = f - nu_c with |d| >= r / eta,
 //     sum over the cluster = (1/d) sum_{k>=1} Q_k (r/d)^k,
 //     Q_k = r^-k sum_lines kappa (Im (zeta+ - nu_c)^k + Im (zeta- - nu_c)^k),
-// a real series in the real variable r/d; the terms beyond k = 24 add at most
-// ~25 eta^24 = 1e-6 (eta = 1/2) relative to the cluster's sum, since each line's
-// Im (zeta - nu_c)^k is gamma times a polynomial of degree k - 1.
+// a real series in the real variable r/d.  Each line's Im (zeta - nu_c)^k is gamma times
+// a polynomial of degree k - 1, so the terms beyond k = 24 add at most
+// 25 rho^24 (1 + rho)^2 / (1 - rho)^2 of the cluster's sum (rho = r/|d|); the planner
+// holds that below 5e-7 (rho <= 0.43).
 //
-// When every line is broad (Lorentz widths >= 1 cm^-1), the radius of a real-centred
-// cluster is at least its largest gamma and no cluster serves bins within twice that
-// distance.  Such points use a complex centre c = nu_c + i gamma_c instead:
+// The radius of a real-centred cluster is at least its largest gamma, so with broad lines
+// no cluster serves bins within about twice that distance.  Points whose lines are all
+// broad (Lorentz widths >= 0.05 cm^-1, where the gamma_min of the bound below stays
+// moderate) use a complex centre c = nu_c + i gamma_c instead:
 //     sum over the cluster = Re[ -i (1/D) sum_{k=0..23} M_k (r/D)^k ],  D = f - c,
 //     M_k = sum_lines kappa ((zeta+ - c)^k + (zeta- - c)^k) / r^k,
 // where r bounds |zeta - c|.  A line's share is kappa gamma / |zeta - f|^2, and the
@@ -77,6 +79,28 @@ __global__ __launch_bounds__(256) void k_far_kind(MixedPlan P) {
     P.far_kind[blockIdx.x] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3])) >= kFarCplxGamma ? 1 : 0;
 }
 
+// 64-bit DPP move (every source lane valid for the controls used here).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Maximum (MAX) or minimum over the wave, in every lane: quad swaps and rotations by 4 and
+// 8 inside each row of 16 lanes, then across rows.
+template <bool MAX>
+__device__ __forceinline__ double wave_extreme(double v) {
+  auto pick = [](double a, double b) { return MAX ? fmax(a, b) : fmin(a, b); };
+  v = pick(v, dpp_f64<0xB1>(v));          // quad_perm [1, 0, 3, 2]
+  v = pick(v, dpp_f64<0x4E>(v));          // quad_perm [2, 3, 0, 1]
+  v = pick(v, dpp_f64<0x124>(v));         // row_ror 4
+  v = pick(v, dpp_f64<0x128>(v));         // row_ror 8
+  v = pick(v, __shfl_xor(v, 16));
+  return pick(v, __shfl_xor(v, 32));
+}
+
 // Sums over the 64 lanes of N values per lane (N a multiple of 8), by halving: at lane
 // distance 32, 16 and 8 each lane keeps half of its values and adds the partner's copy of
 // that half, then the N/8 values left are summed over the 8 lanes (distances 4, 2, 1).
@@ -125,19 +149,16 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   const double al = act ? 1.0 / g : 0.0;
   const double s = kPoleShift * al, im = y * al;               // im: the line's gamma
   const double dp = nu + s - c, dm = nu - s - c;
-  double amax = al, ymin = act ? y : 1e300, glo = act ? im : 1e300, ghi = act ? im : -1e300;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    amax = fmax(amax, __shfl_xor(amax, o));
-    ymin = fmin(ymin, __shfl_xor(ymin, o));
-    glo = fmin(glo, __shfl_xor(glo, o));
-    ghi = fmax(ghi, __shfl_xor(ghi, o));
+  const double amax = wave_extreme<true>(al);
+  const double ymin = wave_extreme<false>(act ? y : 1e300);
+  double glo = 0.0, ghi = 0.0;                                 // gamma range: complex centres only
+  if (cplx) {
+    glo = wave_extreme<false>(act ? im : 1e300);
+    ghi = wave_extreme<true>(act ? im : -1e300);
   }
   const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
   const double wim = im - gc;                                  // Im (zeta - centre)
-  double r2 = act ? fmax(dp * dp, dm * dm) + wim * wim : 0.0;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
+  const double r2 = wave_extreme<true>(act ? fmax(dp * dp, dm * dm) + wim * wim : 0.0);
   double r = sqrt(r2);
   if (!(r > 0.0)) r = 1.0;
   const double ir = 1.0 / r;
@@ -157,17 +178,20 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   const double wrp = dp * ir, wrm = dm * ir;
   const int first = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
   if (!cplx) {
+    // Im w^k of each pole by the real recurrence z_k = 2 Re(w) z_(k-1) - |w|^2 z_(k-2)
+    // (w and its conjugate are the roots of t^2 - 2 Re(w) t + |w|^2), z_0 = 0, z_1 = Im w
     double v[kFarP];
-    double pr = wrp, pi = wi, mr = wrm, mi = wi;               // w+^1, w-^1
+    const double tp = 2.0 * wrp, np = wrp * wrp + wi * wi;
+    const double tm = 2.0 * wrm, nm = wrm * wrm + wi * wi;
+    double p1 = wi, p0 = 0.0, m1 = wi, m0 = 0.0;
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) {
-      v[k] = kap * pi + kap * mi;
-      const double nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
-      const double n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
-      pr = nr;
-      pi = ni;
-      mr = n2r;
-      mi = n2i;
+      v[k] = kap * p1 + kap * m1;
+      const double p2 = tp * p1 - np * p0, m2 = tm * m1 - nm * m0;
+      p0 = p1;
+      p1 = p2;
+      m0 = m1;
+      m1 = m2;
     }
     wave_sums_f64(v, lane);
     if ((lane & 7) == 0) {
@@ -317,6 +341,29 @@ __device__ __forceinline__ int lower_bound_int(const int* __restrict__ v, int a,
   return a;
 }
 
+// Whether a series of P orders about centre nu_c (+ i gamma_c), radius r, converges within
+// kFarTol relative to the sum at every bin at least dist from nu_c.  With rho = r / |D| the
+// tail of a line's series is, relative to the line's own value: real centres (each term is
+// gamma times a polynomial of degree k - 1 in w, |w| <= 1)
+//     (P + 1) rho^P (1 + rho)^2 / (1 - rho)^2,
+// complex centres (the line's value is kappa gamma / |zeta - f|^2 >= kappa gamma /
+// (|D| (1 + rho))^2 and the tail at most kappa rho^P / ((1 - rho) |D|))
+//     rho^P (1 + rho)^2 |D| / ((1 - rho) gamma_min).
+// Both fall as |D| grows, so the nearest bin decides.
+template <int P>
+__device__ __forceinline__ bool series_ok(double r, double gamma_c, double gamma_min, double dist, bool cplx) {
+#pragma clang fp contract(off)
+  const double D = cplx ? sqrt(dist * dist + gamma_c * gamma_c) : dist;
+  const double rho = r / D;
+  if (!(rho <= kFarEta)) return false;
+  double rp = 1.0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) rp *= rho;
+  const double w = (1.0 + rho) * (1.0 + rho);
+  return cplx ? rp * w * D <= kFarTol * (1.0 - rho) * gamma_min
+              : (double)(P + 1) * rp * w <= kFarTol * (1.0 - rho) * (1.0 - rho);
+}
+
 // Whether cluster (geometry gm, record rc) serves every bin of [f_lo, f_hi]: no line of it
 // has its W4 core there (its lowest y is >= 15, or its lines are 15 Doppler widths away)
 // and its series converges there within the truncation bound of its kind.
@@ -326,12 +373,7 @@ __device__ __forceinline__ bool serves(const double4 gm, const double* __restric
   const double dist = fmax(fmax(f_lo - gm.x, gm.x - f_hi), 0.0);      // centre to the bins
   const double edge = fmax(fmax(f_lo - gm.z, gm.y - f_hi), 0.0);      // the lines' nu range to the bins
   if (!(rc[2] >= 15.0 || edge >= 15.001 * rc[1])) return false;
-  if (!cplx) return rc[0] <= kFarEta * dist;
-  const double D = sqrt(dist * dist + rc[5] * rc[5]);
-  const double rho = rc[0] / D;
-  if (!(rho <= kFarEta)) return false;
-  const double r2 = rho * rho, r4 = r2 * r2, r8 = r4 * r4;
-  return r8 * r8 * r8 * (1.0 + rho) * (1.0 + rho) * D <= kFarTol * (1.0 - rho) * rc[3];
+  return series_ok<kFarP>(rc[0], rc[5], rc[3], dist, cplx);
 }
 
 // Sum of a cluster (record rc, centre nu_c) at bin f.
@@ -526,22 +568,39 @@ __device__ __forceinline__ int upper_bound_int(const int* __restrict__ v, int a,
 
 // Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
 // kernel's: c y (u + 1) / (u^2 + 2 y^2), u = x^2 + y^2 - 1/2, scaled by amp = S g).
+__device__ __forceinline__ double region1_line(double nu, double g, double y, float amp, double f) {
+#pragma clang fp contract(off)
+  const double x = (nu_if(nu) - f) * g;
+  const double y2 = y * y;
+  const double u = x * x + y2 - 0.5;
+  return (double)amp * 0.5641896 * y * (u + 1.0) / (u * u + 2.0 * y2);
+}
+
 __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedTask& tk, int a, int z, double f) {
 #pragma clang fp contract(off)
   double s = 0.0;
-  for (int l = a; l < z; ++l) {
-    const double x = (nu_if(P.nu[l]) - f) * tk.g[l];
-    const double y = tk.y[l];
-    const double y2 = y * y;
-    const double u = x * x + y2 - 0.5;
-    s += (double)tk.amp[l] * 0.5641896 * y * (u + 1.0) / (u * u + 2.0 * y2);
+  int l = a;
+  // four lines' loads issued together, then their terms added in line order
+  for (; l + 4 <= z; l += 4) {
+    double nu[4], g[4], y[4];
+    float amp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      nu[i] = P.nu[l + i];
+      g[i] = tk.g[l + i];
+      y[i] = tk.y[l + i];
+      amp[i] = tk.amp[l + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s += region1_line(nu[i], g[i], y[i], amp[i], f);
   }
+  for (; l < z; ++l) s += region1_line(P.nu[l], tk.g[l], tk.y[l], tk.amp[l], f);
   return s;
 }
 
 // Region I of the lines [a, z) (every one contributing) at bin f: leaves of the range
 // from the largest aligned clusters that serve f, partial leaves line by line.
-__device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, bool cplx,
+__device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, bool cplx,
                             int a, int z, double f) {
 #pragma clang fp contract(off)
   if (z <= a) return 0.0;
@@ -571,43 +630,244 @@ __device__ double range_sum(const MixedPlan& P, const MixedTask& tk, const doubl
   return s + region1_lines(P, tk, max(B * kFarLeaf, a), z, f);
 }
 
-// Window-end lines, one bin per lane (a block is 64 R bins, so every wave lies in one
-// block): bin j gets the lines ending inside its block with ie > j and those starting
-// inside it with ib <= j.
-template <int R>
-__global__ __launch_bounds__(256) void k_far_edges(MixedPlan P) {
+// ---- window-end ranges from one expansion per (point, block, range) ----
+//
+// A window-end range's lines lie together, a cutoff away from the block, so one series of
+// kRangeP orders about the range's own centre converges fast at every bin of the block.
+// Its moments come from the leaves' moments shifted to that centre (exact up to rounding:
+// order k of the shifted series takes orders <= k of a leaf's).  Bin j of the range ending
+// inside the block needs the whole leaves [A_j, B) (A_j rises with j): thread j shifts the
+// leaves [A_j, A_j+1) and a suffix sum over the threads, in a fixed order, gives each its
+// [A_j, B); the range starting inside the block is the mirror image (prefix sums).
+
+constexpr int kRangeP = 16;
+
+template <bool CPLX>
+struct RangeMoments {
+  static constexpr int N = CPLX ? 2 * kRangeP : kRangeP;   // real: orders 1..16; complex: (Re, Im) of 0..15
+};
+
+// Leaf moments (record rc, centre nu_l) shifted to the centre cE + i gE with radius rE,
+// the first kRangeP orders added to m.
+template <bool CPLX>
+__device__ __forceinline__ void leaf_to_range(const double* __restrict__ rc, double nu_l, double cE, double gE,
+                                              double rE, double (&m)[RangeMoments<CPLX>::N]) {
 #pragma clang fp contract(off)
-  const int lane = threadIdx.x & 63;
-  const int t = blockIdx.x * 256 + threadIdx.x;                // bin of the chunk
-  const int b = __builtin_amdgcn_readfirstlane(t / (64 * R));
-  if (b >= P.nblocks) return;
+  const double b = rc[0] / rE;
+  if (!CPLX) {
+    const double a = (nu_l - cE) / rE;
+    double v[kRangeP + 1];
+    v[0] = 0.0;                                                // Q_0 of a real centre is 0
+    double bm = b;
+#pragma unroll
+    for (int k = 1; k <= kRangeP; ++k) {
+      v[k] = bm * rc[kFarHead + k - 1];
+      bm *= b;
+    }
+#pragma unroll
+    for (int j = 1; j <= kRangeP; ++j)
+#pragma unroll
+      for (int k = kRangeP; k >= j; --k) v[k] = v[k] + a * v[k - 1];
+#pragma unroll
+    for (int k = 1; k <= kRangeP; ++k) m[k - 1] += v[k];
+  } else {
+    const double ar = (nu_l - cE) / rE, ai = (rc[5] - gE) / rE;
+    double vr[kRangeP], vi[kRangeP];
+    double bm = 1.0;
+#pragma unroll
+    for (int k = 0; k < kRangeP; ++k) {
+      vr[k] = bm * rc[kFarHead + 2 * k];
+      vi[k] = bm * rc[kFarHead + 2 * k + 1];
+      bm *= b;
+    }
+#pragma unroll
+    for (int j = 1; j < kRangeP; ++j) {
+#pragma unroll
+      for (int k = kRangeP - 1; k >= j; --k) {
+        const double nr = vr[k] + (ar * vr[k - 1] - ai * vi[k - 1]);
+        const double ni = vi[k] + (ar * vi[k - 1] + ai * vr[k - 1]);
+        vr[k] = nr;
+        vi[k] = ni;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kRangeP; ++k) {
+      m[2 * k] += vr[k];
+      m[2 * k + 1] += vi[k];
+    }
+  }
+}
+
+// The series of m (centre cE + i gE, radius rE) at bin f.
+template <bool CPLX>
+__device__ __forceinline__ double range_value(const double (&m)[RangeMoments<CPLX>::N], double cE, double gE,
+                                              double rE, double f) {
+#pragma clang fp contract(off)
+  if (!CPLX) {
+    const double id = 1.0 / (f - cE);
+    const double t = rE * id;
+    double q = m[kRangeP - 1];
+#pragma unroll
+    for (int k = kRangeP - 2; k >= 0; --k) q = q * t + m[k];
+    return q * t * id;
+  }
+  const double dr = f - cE, di = -gE;
+  const double inv = 1.0 / (dr * dr + di * di);
+  const double ir = dr * inv, ii = -di * inv;
+  const double tr = rE * ir, ti = rE * ii;
+  double sr = m[2 * (kRangeP - 1)], si = m[2 * (kRangeP - 1) + 1];
+#pragma unroll
+  for (int k = kRangeP - 2; k >= 0; --k) {
+    const double nr = (sr * tr - si * ti) + m[2 * k];
+    const double ni = (sr * ti + si * tr) + m[2 * k + 1];
+    sr = nr;
+    si = ni;
+  }
+  return sr * ii + si * ir;
+}
+
+// Maximum (MAX) or minimum of v over the workgroup of NW waves; every thread gets it.
+template <int NW, bool MAX>
+__device__ __forceinline__ double block_extreme(double v, double* s_red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o);
+    v = MAX ? fmax(v, w) : fmin(v, w);
+  }
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  v = s_red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) v = MAX ? fmax(v, s_red[w]) : fmin(v, s_red[w]);
+  __syncthreads();
+  return v;
+}
+
+// Suffix (SUFFIX) or prefix sums of N values over the threads of the workgroup, in a fixed
+// order: within each wave by doubling distances, then the totals of the other waves.
+template <int N, int NW, bool SUFFIX>
+__device__ __forceinline__ void block_scan(double (&v)[N], double* s_tot) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const bool take = SUFFIX ? lane + d < 64 : lane >= d;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double x = SUFFIX ? __shfl_down(v[i], d) : __shfl_up(v[i], d);
+      if (take) v[i] += x;
+    }
+  }
+  if (lane == (SUFFIX ? 0 : 63)) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) s_tot[wave * N + i] = v[i];
+  }
+  __syncthreads();
+  double add[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) add[i] = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    if (SUFFIX ? w > wave : w < wave) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) add[i] += s_tot[w * N + i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += add[i];
+  __syncthreads();
+}
+
+// Whole leaves [lo, hi) of thread's range at bin f (every thread of the workgroup calls
+// this together): SUFFIX, hi is the same for all threads and lo rises with the thread;
+// else lo is the same and hi rises.  From one shifted series when it meets the truncation
+// bound at every bin of the block, else through the tree bin by bin.
+template <int NW, bool CPLX, bool SUFFIX>
+__device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, int lo,
+                             int hi, double f, double f_lo, double f_hi, int* s_cut, double* s_tot, double* s_red) {
+#pragma clang fp contract(off)
+  constexpr int NB = 64 * NW;
+  const int tid = threadIdx.x;
+  __syncthreads();                                             // s_cut of an earlier call read
+  s_cut[tid] = SUFFIX ? lo : hi;
+  __syncthreads();
+  const int U0 = SUFFIX ? s_cut[0] : lo;                       // the union of the threads' leaves
+  const int U1 = SUFFIX ? hi : s_cut[NB - 1];
+  const int own0 = SUFFIX ? lo : (tid ? s_cut[tid - 1] : lo);  // the leaves this thread shifts
+  const int own1 = SUFFIX ? (tid + 1 < NB ? s_cut[tid + 1] : hi) : hi;
+  if (U0 >= U1) return 0.0;
+  const double cE = 0.5 * (P.far_geom[U0].y + P.far_geom[U1 - 1].z);
+  double glo = 1e300, ghi = -1e300;
+  for (int L = U0 + tid; L < U1; L += NB) {
+    const double* rc = recs + (size_t)L * kFarRec;
+    glo = fmin(glo, rc[3]);
+    ghi = fmax(ghi, rc[4]);
+  }
+  glo = block_extreme<NW, false>(glo, s_red);
+  ghi = block_extreme<NW, true>(ghi, s_red);
+  const double gE = CPLX ? 0.5 * (glo + ghi) : 0.0;
+  double rE = 0.0;
+  for (int L = U0 + tid; L < U1; L += NB) {
+    const double* rc = recs + (size_t)L * kFarRec;
+    const double dx = P.far_geom[L].x - cE, dy = rc[5] - gE;
+    rE = fmax(rE, (CPLX ? sqrt(dx * dx + dy * dy) : fabs(dx)) + rc[0]);
+  }
+  rE = block_extreme<NW, true>(rE, s_red);
+  if (!(rE > 0.0)) rE = 1.0;
+  const double dist = fmax(fmax(f_lo - cE, cE - f_hi), 0.0);
+  if (!series_ok<kRangeP>(rE, gE, glo, dist, CPLX))
+    return range_sum(P, tk, recs, CPLX, lo * kFarLeaf, hi * kFarLeaf, f);
+  double m[RangeMoments<CPLX>::N];
+#pragma unroll
+  for (int i = 0; i < RangeMoments<CPLX>::N; ++i) m[i] = 0.0;
+  for (int L = own0; L < own1; ++L) leaf_to_range<CPLX>(recs + (size_t)L * kFarRec, P.far_geom[L].x, cE, gE, rE, m);
+  block_scan<RangeMoments<CPLX>::N, NW, SUFFIX>(m, s_tot);
+  return range_value<CPLX>(m, cE, gE, rE, f);
+}
+
+// Window-end lines, one workgroup per (task, block), one bin per thread: bin j gets the
+// lines ending inside the block with ie > j and those starting inside it with ib <= j,
+// each range as at most one partial leaf line by line plus whole leaves (leaves_sum).
+template <int R, bool CPLX>
+__global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
+#pragma clang fp contract(off)
+  constexpr int NB = 64 * R;
+  __shared__ int s_cut[NB];
+  __shared__ double s_tot[R * RangeMoments<CPLX>::N];
+  __shared__ double s_red[R];
+  const int b = blockIdx.x;
   const int task = blockIdx.y;
   const size_t slot = (size_t)task * P.nblocks + b;
   const int4 er = P.edge_rng[slot];
   if (er.x >= er.y && er.z >= er.w) return;                    // set for whole blocks only
+  if ((P.far_kind[task] != 0) != CPLX) return;                 // the other kind's launch
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
   const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
-  const bool cplx = P.far_kind[task] != 0;
+  const int w0 = gv.start_idx + b * NB;
+  const int j = w0 + threadIdx.x;
+  const double f = gv.freq[j], f_lo = gv.freq[w0], f_hi = gv.freq[w0 + NB - 1];
   unsigned long long pairs = 0;
-  const int j = gv.start_idx + t;
-  const double f = gv.freq[j];
   double acc = 0.0;
-  if (er.x < er.y) {
+  if (er.x < er.y) {                                           // er.y is a leaf boundary
     const int s0 = upper_bound_int(P.ie, er.x, er.y, j);       // ie > j from here
-    acc += range_sum(P, tk, recs, cplx, s0, er.y, f);
+    const int A = (s0 + kFarLeaf - 1) / kFarLeaf;
     pairs += (unsigned long long)(er.y - s0);
+    acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
+    acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red);
   }
-  if (er.z < er.w) {
+  if (er.z < er.w) {                                           // er.z is a leaf boundary
     const int p1 = upper_bound_int(P.ib, er.z, er.w, j);       // ib <= j before here
-    acc += range_sum(P, tk, recs, cplx, er.z, p1, f);
+    const int D = p1 / kFarLeaf;
     pairs += (unsigned long long)(p1 - er.z);
+    acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
+    acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, f_lo, f_hi, s_cut, s_tot, s_red);
   }
   tk.out[j] += acc;
   // pairs of this wave, counted once
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
-  if (lane == 0 && pairs) atomicAdd(P.far_evals, pairs);
+  if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(P.far_evals, pairs);
 }
 
 }  // namespace
@@ -628,13 +888,15 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
 hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
-  const dim3 grid_bins((unsigned)(((long long)plan.nblocks * 64 * R + 255) / 256), plan.ntasks);
+  const dim3 grid_blocks(plan.nblocks, plan.ntasks);
   if (R == 4) {
     k_far_eval<4><<<grid, 256, 0, s>>>(plan);
-    k_far_edges<4><<<grid_bins, 256, 0, s>>>(plan);
+    k_far_edges<4, false><<<grid_blocks, 256, 0, s>>>(plan);
+    k_far_edges<4, true><<<grid_blocks, 256, 0, s>>>(plan);
   } else {
     k_far_eval<8><<<grid, 256, 0, s>>>(plan);
-    k_far_edges<8><<<grid_bins, 256, 0, s>>>(plan);
+    k_far_edges<8, false><<<grid_blocks, 512, 0, s>>>(plan);
+    k_far_edges<8, true><<<grid_blocks, 512, 0, s>>>(plan);
   }
   return hipGetLastError();
 }

@@ -85,13 +85,13 @@ struct MixedTask {
 };
 
 // Far field of region I: moments per cluster, leaf size, list length, levels.
-constexpr int kFarP = 24;            // moments per cluster (truncation ~ 25 eta^24 relative, real centres)
+constexpr int kFarP = 24;            // moments per cluster
 constexpr double kFarEta = 0.5;      // a cluster of radius r serves bins at distance >= r / eta
 constexpr int kFarLeaf = 64;         // lines per leaf
 constexpr int kFarHead = 6;          // radius, max 1/g, min y, lowest and highest gamma, gamma of the centre
 constexpr int kFarRec = kFarHead + 2 * kFarP;   // head, then P real or P complex moments
-constexpr double kFarTol = 5e-7;     // bound on the relative truncation error of complex-centred sums
-constexpr double kFarCplxGamma = 1.0;   // points whose sampled Lorentz widths are all >= this (cm^-1)
+constexpr double kFarTol = 5e-7;     // bound on the relative truncation error of every far-field sum
+constexpr double kFarCplxGamma = 0.05;  // points whose sampled Lorentz widths are all >= this (cm^-1)
                                         // take complex-centred moments
 constexpr int kFarList = 128;        // clusters per block at most
 constexpr int kFarMaxLevels = 32;

@@ -362,8 +362,9 @@ struct MixedShape {
 //   [0]      ib, ie, k0, k1 (ints: window and core ranges, absolute bins); anchored fast
 //            lines keep x at the middles of slices 0-3 here instead (they need no ranges)
 //   [1]      e, h, k, a      e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
-//   [2]      b, r, cf, -     b = a (scaled); r = x at the centre bin; cf: centre bin
-//                            relative to the wave's first bin
+//   [2]      b, r, cf, s     b = a (scaled); r = x at the centre bin; cf: centre bin
+//                            relative to the wave's first bin; s: slice codes of an edge
+//                            line (int bits)
 //   [3 ...]  anchored fast lines: x at the middles of slices 4-7 (R = 8)
 // so the fast loop reads everything of a line from one LDS address.
 template <int R>
@@ -377,6 +378,11 @@ struct StagedRec {
 // |x| + y >= 15 the relative error of the profile stays below eps (1 + 3e); the form is
 // used for lines with e = dnu/alpha <= 8 (25 eps), the exact-integer form elsewhere.
 constexpr double kAnchorMaxE = 8.0;
+
+// Slice codes of an edge line (2 bits per 64-bin slice of the wave's bins).
+constexpr int kSliceMiss = 0;   // the window misses the slice
+constexpr int kSliceFull = 1;   // the window covers the slice and the core misses it
+constexpr int kSlicePart = 2;   // per-lane window and core test
 
 // Kahan summation with the compensation kept in the running partial sum: part holds
 // the terms added since the last step plus the rounding error of that step (Fast2Sum
@@ -538,6 +544,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
         } else {
           rec[0] = make_float4(__int_as_float(ib), __int_as_float(ie), __int_as_float(core.x),
                                __int_as_float(core.y));
+          // slice codes of an edge line, 2 bits per 64-bin slice
+          int codes = 0;
+#pragma unroll
+          for (int q = 0; q < R; ++q) {
+            const int s0 = w0 + 64 * q, s1 = s0 + 64;
+            const int slo = max(ib, s0), shi = min(ie, s1);
+            const int c = slo >= shi ? kSliceMiss
+                          : (slo == s0 && shi == s1 && (core.y <= core.x || core.y <= s0 || core.x >= s1))
+                              ? kSliceFull
+                              : kSlicePart;
+            codes |= c << (2 * q);
+          }
+          rec[2].w = __int_as_float(codes);
         }
       }
       // Compact the chunk's records by class, in line order: anchored fast lines first, then
@@ -584,19 +603,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
             part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
         }
       });
-      // edge lines: per-lane window test; the line's core bins [k0, k1) belong to the core pass
+      // edge lines: per 64-bin slice the staged slice code says whether the line misses it
+      // (skipped), covers it clear of its core (no per-lane test) or covers part of it (the
+      // per-lane window and core test); the core bins [k0, k1) belong to the core pass
       run_block(ne, [&](int a, int z) {
         for (int k = na + nx + a; k < na + nx + z; ++k) {
           const float4 Iv = my_l[k][0];
-          const int4 I = make_int4(__float_as_int(Iv.x), __float_as_int(Iv.y), __float_as_int(Iv.z),
-                                   __float_as_int(Iv.w));
           const float4 E = my_l[k][1];
           const float4 B = my_l[k][2];
+          const int codes = __builtin_amdgcn_readfirstlane(__float_as_int(B.w));
 #pragma unroll
           for (int q = 0; q < R; ++q) {
-            const int jj = w0 + lane + 64 * q;
-            if (jj >= I.x && jj < I.y && (jj < I.z || jj >= I.w))
-              part[q] += region1(__builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E.y, E.z, E.w, B.x);
+            const int c = (codes >> (2 * q)) & 3;
+            if (c == kSliceMiss) continue;
+            const float x = __builtin_fmaf(B.z - lanef_q[q], E.x, B.y);
+            const float u = __builtin_fmaf(x, x, E.y);
+            const float rden = __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, E.z));
+            float num = __builtin_fmaf(E.w, u, B.x);
+            if (c == kSlicePart) {
+              const int jj = w0 + lane + 64 * q;
+              const bool ok = jj >= __float_as_int(Iv.x) && jj < __float_as_int(Iv.y) &&
+                              (jj < __float_as_int(Iv.z) || jj >= __float_as_int(Iv.w));
+              num = ok ? num : 0.0f;
+            }
+            part[q] = __builtin_fmaf(num, rden, part[q]);
           }
         }
       });

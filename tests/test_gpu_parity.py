@@ -147,8 +147,8 @@ def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
 def test_voigt_mixed_far_field(voigt_lines, monkeypatch, T, P):
     """Region I from far-field moments (the default) against the FP64 oracle, against the
     same run with the far field off (GX_FAR=0), and engaged: a share of the pairs came
-    from moments.  At 100 atm every Lorentz width is >= 1 cm^-1 and the moments are
-    complex-centred."""
+    from moments.  At 10 and 100 atm every Lorentz width is >= 0.05 cm^-1 and the moments
+    are complex-centred."""
     ll = voigt_lines
     Q = po.partition(ll.state_e, ll.state_g, T)
     sel = np.nonzero((ll.nu > 5900.0) & (ll.nu < 6300.0))[0]
