@@ -703,17 +703,10 @@ __device__ __forceinline__ float dpp_take(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
 }
 
-// Inclusive scan within rows of 16 (row_shr 1, 2, 4, 8), then the row totals
-// carried by row_bcast 15 and 31: lane 63 ends with the sum of all 64 lanes,
-// always added in the same order.
-__device__ __forceinline__ float wave_sum_lane63(float v) {
-  v += dpp_take<0x111, 0xf>(v);
-  v += dpp_take<0x112, 0xf>(v);
-  v += dpp_take<0x114, 0xf>(v);
-  v += dpp_take<0x118, 0xf>(v);
-  v += dpp_take<0x142, 0xa>(v);
-  v += dpp_take<0x143, 0xc>(v);
-  return v;
+// row_shr within rows of 16, 0 where the source lane is outside the row
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -773,12 +766,17 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
     if (!__ballot(has)) continue;
     const int kmin = max(wave_min(has ? core.x - cj : 0x3fffffff), -P.kmax);
     const int kend = min(wave_max(has ? core.y - cj : -0x3fffffff), P.kmax + 1);
+    // segments: lanes with equal centre bins (contiguous, nu is sorted); each lane's distance
+    // from its segment's first lane fixes which steps of the scan below it takes
     const int cprev = __shfl_up(cj, 1), cnext = __shfl_down(cj, 1);
     const bool head = lane == 0 || cprev != cj;
     const bool tail = lane == 63 || cnext != cj;
-    const bool single = head && tail;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long multi = __ballot(head && !tail);
+    const unsigned long long upto = __ballot(head) & ((2ull << lane) - 1ull);
+    const int hd = lane - (63 - __builtin_clzll(upto));
+    const float m1 = hd >= 1 ? 1.0f : 0.0f, m2 = hd >= 2 ? 1.0f : 0.0f, m4 = hd >= 4 ? 1.0f : 0.0f;
+    const float m8 = hd >= 8 ? 1.0f : 0.0f;
+    const float m15 = hd >= (lane & 15) + 1 ? 1.0f : 0.0f;   // lane 16 row - 1 is in my segment
+    const float m31 = hd >= lane - 31 ? 1.0f : 0.0f;          // lane 31 is in my segment
     for (int k = kmin; k < kend; ++k) {
       const int j = cj + k;
       float v = 0.0f;
@@ -787,20 +785,16 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
         const double x = fabs(nu - fq[j - fq_off]) * g;
         v = amp * w4_core_mixed(x, y);
       }
-      if (single && on) atomicAdd(&acc[j - base], (double)v);   // one lane per address: a plain add, no wait
-      const unsigned long long onm = __ballot(on);
-      unsigned long long h = multi & (onm ? ~0ull : 0ull);
-      while (h) {
-        const int a = __builtin_ctzll(h);
-        h &= h - 1;
-        const unsigned long long after = heads & ~((2ull << a) - 1ull);   // heads after lane a
-        const int e = after ? __builtin_ctzll(after) : 64;
-        const unsigned long long seg = (e == 64 ? ~0ull : (1ull << e) - 1ull) & ~((1ull << a) - 1ull);
-        if (!(onm & seg)) continue;
-        const float tot = wave_sum_lane63(lane >= a && lane < e ? v : 0.0f);
-        const int ia = __builtin_amdgcn_readlane(cj, a) + k - base;
-        if (lane == 63 && (unsigned)ia < (unsigned)span) atomicAdd(&acc[ia], (double)tot);
-      }
+      // segmented inclusive sum in a fixed order (rows of 16 by row_shr 1/2/4/8, then the
+      // row totals by row_bcast 15/31): each segment's last lane ends with its total
+      v = __builtin_fmaf(dpp_shr<0x111>(v), m1, v);
+      v = __builtin_fmaf(dpp_shr<0x112>(v), m2, v);
+      v = __builtin_fmaf(dpp_shr<0x114>(v), m4, v);
+      v = __builtin_fmaf(dpp_shr<0x118>(v), m8, v);
+      v = __builtin_fmaf(dpp_take<0x142, 0xa>(v), m15, v);
+      v = __builtin_fmaf(dpp_take<0x143, 0xc>(v), m31, v);
+      const int ia = j - base;
+      if (tail && v != 0.0f && (unsigned)ia < (unsigned)span) atomicAdd(&acc[ia], (double)v);   // one lane per address
     }
   }
   __syncthreads();
