@@ -476,10 +476,12 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
 // region-I kernel.  The end of each range that is the same for every bin of the block (the
 // last line ending inside it, the first line starting inside it) is moved to a leaf
 // boundary, the lines beyond it staying with the region-I kernel, so k_far_edges evaluates
-// at most one partial leaf per range and bin.  One thread per (task, block).
+// at most one partial leaf per range and bin.  One wave per (task, block), the lanes testing
+// the leaves side by side.
 __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
 #pragma clang fp contract(off)
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (b >= P.nblocks) return;
   const int task = blockIdx.y;
   const size_t slot = (size_t)task * P.nblocks + b;
@@ -498,10 +500,11 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
     const int ep0 = (max(L1, L2) + kFarLeaf - 1) / kFarLeaf * kFarLeaf, ep1 = Ly;
     const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
     const bool cplx = P.far_kind[task] != 0;
-    auto leaves_usable = [&](int a, int z) -> bool {
-      for (int leaf = a / kFarLeaf; leaf * kFarLeaf < z; ++leaf)
-        if (!serves(P.far_geom[leaf], recs + (size_t)leaf * kFarRec, cplx, f_lo, f_hi)) return false;
-      return true;
+    auto leaves_usable = [&](int a, int z) -> bool {       // every lane calls it together
+      bool ok = true;
+      for (int leaf = a / kFarLeaf + lane; ok && leaf * kFarLeaf < z; leaf += 64)
+        ok = serves(P.far_geom[leaf], recs + (size_t)leaf * kFarRec, cplx, f_lo, f_hi);
+      return __ballot(!ok) == 0ull;
     };
     if (em1 > em0 && leaves_usable(em0, em1)) {
       er.x = em0;
@@ -512,7 +515,7 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
       er.w = ep1;
     }
   }
-  P.edge_rng[slot] = er;
+  if (lane == 0) P.edge_rng[slot] = er;
 }
 
 // Far-field sums of one (task, block) per wave, added to the task's grid.
@@ -881,7 +884,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
   k_far_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
-  k_edge_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
+  k_edge_plan<<<dim3((plan.nblocks + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
   return hipGetLastError();
 }
 

@@ -187,29 +187,43 @@ __global__ __launch_bounds__(256) void k_windows(WindowArgs wa, GridView gv, con
 // path's own region-I test (evaluated on the grid array itself).  The set is
 // an interval (|nu - f_j| is V-shaped in j); it is found from a bracketing
 // estimate one bin wider on each side and trimmed with the exact test.
-// blockIdx.y = point.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool in_core(double nu, double g, double y, const double* freq, int j) {
 #pragma clang fp contract(off)
   return fabs(nu - freq[j]) * g + y < 15.0;
 }
 
-__device__ __forceinline__ unsigned long long prep_mixed_line(const PointConsts& pc, const LinesView& lv,
-                                                              const GridView& gv, const int* __restrict__ IB,
-                                                              const int* __restrict__ IE, const MixedLineData& out,
-                                                              int i, int p) {
+// The temperature-dependent part of one line at one point: S, 1/alpha and (T_ref/T)^n.
+struct PrepT {
+  double S, g, tscale;
+};
+
+__device__ __forceinline__ PrepT prep_mixed_t(const PointConsts& pc, double nu_if, double a, int gi, double elow,
+                                              double nexp) {
 #pragma clang fp contract(off)
-  const double nu = lv.nu[i];
-  const double nu_if = nu == 0 ? 1e-6 : nu;
-  const double a = lv.aif[i];
-  const int gi = lv.gns[i];
-  double S = pc.cmcoef * a * gi * kIsqrtPi * exp(-pc.beta * lv.elow[i]) * (1.0 - exp(-pc.beta * nu_if)) /
-             (nu_if * nu_if * pc.partition);
-  if (pc.convention == 1 && gi == -1) S = a;
-  const double g = 1.0 / (nu_if * pc.dpwcoef);
-  const double gammaL = (lv.gamma[i] * pc.pressure_scale) * pow(pc.ref_temp / pc.temperature, lv.nexp[i]);
+  PrepT t;
+  t.S = pc.cmcoef * a * gi * kIsqrtPi * exp(-pc.beta * elow) * (1.0 - exp(-pc.beta * nu_if)) /
+        (nu_if * nu_if * pc.partition);
+  if (pc.convention == 1 && gi == -1) t.S = a;
+  t.g = 1.0 / (nu_if * pc.dpwcoef);
+  t.tscale = pow(pc.ref_temp / pc.temperature, nexp);
+  return t;
+}
+
+// Whether two points share every constant prep_mixed_t reads.
+__device__ __forceinline__ bool same_t(const PointConsts& a, const PointConsts& b) {
+  return a.beta == b.beta && a.cmcoef == b.cmcoef && a.partition == b.partition && a.dpwcoef == b.dpwcoef &&
+         a.ref_temp == b.ref_temp && a.temperature == b.temperature && a.convention == b.convention;
+}
+
+// The pressure-dependent part and the outputs: y, amp, the core range.
+__device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts& pc, const PrepT& t, double nu_if,
+                                                               double gamma, int ib, int ie, const GridView& gv,
+                                                               const MixedLineData& out, int i, int p) {
+#pragma clang fp contract(off)
+  const double g = t.g;
+  const double gammaL = (gamma * pc.pressure_scale) * t.tscale;
   const double y = gammaL * g;
-  const int ib = IB[i], ie = IE[i];
   int k0 = 0, k1 = 0;
   if (y < 15.0 && ie > ib) {
     const double half = (15.0 - y) / g;
@@ -224,19 +238,33 @@ __device__ __forceinline__ unsigned long long prep_mixed_line(const PointConsts&
     }
   }
   const size_t o = (size_t)p * out.stride + i;
-  out.amp[o] = (float)(S * g);
+  out.amp[o] = (float)(t.S * g);
   out.y[o] = y;
   out.g[o] = g;
   out.core[o] = make_int2(k0, k1);
   return (unsigned long long)(k1 - k0);
 }
 
-__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, LinesView lv, GridView gv,
-                                                    const int* __restrict__ IB, const int* __restrict__ IE,
-                                                    MixedLineData out, unsigned long long* __restrict__ core_evals) {
+// One thread per line over all npts points: the line's record is read once, and the
+// temperature-dependent part (two exp, one pow) is reused by consecutive points at the same
+// temperature (bit-identical: the same operations on the same operands).
+__global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, int npts, LinesView lv,
+                                                    GridView gv, const int* __restrict__ IB,
+                                                    const int* __restrict__ IE, MixedLineData out,
+                                                    unsigned long long* __restrict__ core_evals) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long w = 0;
-  if (i < lv.n) w = prep_mixed_line(pcs[blockIdx.y], lv, gv, IB, IE, out, i, blockIdx.y);
+  if (i < lv.n) {
+    const double nu = lv.nu[i];
+    const double nu_if = nu == 0 ? 1e-6 : nu;
+    const double a = lv.aif[i], elow = lv.elow[i], gamma = lv.gamma[i], nexp = lv.nexp[i];
+    const int gi = lv.gns[i], ib = IB[i], ie = IE[i];
+    PrepT t = prep_mixed_t(pcs[0], nu_if, a, gi, elow, nexp);
+    for (int p = 0; p < npts; ++p) {
+      if (p > 0 && !same_t(pcs[p], pcs[p - 1])) t = prep_mixed_t(pcs[p], nu_if, a, gi, elow, nexp);
+      w += prep_mixed_point(pcs[p], t, nu_if, gamma, ib, ie, gv, out, i, p);
+    }
+  }
   if (core_evals) block_count(w, core_evals);
 }
 
@@ -880,7 +908,7 @@ hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView
                              const int* ib, const int* ie, MixedLineData out, unsigned long long* core_evals,
                              hipStream_t s) {
   if (lv.n <= 0 || npts <= 0) return hipSuccess;
-  k_prep_mixed<<<dim3((lv.n + 255) / 256, npts), 256, 0, s>>>(d_pcs, lv, gv, ib, ie, out, core_evals);
+  k_prep_mixed<<<(lv.n + 255) / 256, 256, 0, s>>>(d_pcs, npts, lv, gv, ib, ie, out, core_evals);
   return hipGetLastError();
 }
 
