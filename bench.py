@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compensated add
 # HBM bytes per accumulation launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # same command scaled by a plain-copy calibration (profiles/k_accum_voigt_mixed.table_48.hbm.md)
-PROFILED_TRAFFIC = {"table_48": 4.450e12}
+PROFILED_TRAFFIC = {"table_48": 2.377e11}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -185,7 +185,7 @@ def bench_table(args, wl):
                      "flop_per_eval": FLOP_PER_EVAL},
         "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV)", "ms": core_ms, "evals": core_evals,
                       "evals_per_s": core_evals / (core_ms * 1e-3) if core_ms > 0 else None},
-        "far_field": {"kernels": "k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_far_eval (FP64 moments)",
+        "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, k_far_edges (FP64 moments)",
                       "ms": far_ms, "evals": far_evals, "share": far_evals / max(evals, 1)},
         "intensity_kernels_ms": eager_int,
     }
