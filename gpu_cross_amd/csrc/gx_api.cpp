@@ -250,10 +250,10 @@ void Ranges::release() {
 
 int MixedRunner::init() {
   if (counter) return GX_OK;
-  // GX_MIXED_R (4 or 8) overrides the default bins per lane; used to compare launch shapes
+  // GX_MIXED_R (1, 2, 4 or 8) overrides the default bins per lane; used to compare launch shapes
   if (const char* env = std::getenv("GX_MIXED_R")) {
     const int r = std::atoi(env);
-    if (r == 4 || r == 8) R = r;
+    if (r == 1 || r == 2 || r == 4 || r == 8) R = r;
   }
   if (const char* env = std::getenv("GX_FAR")) far_on = std::atoi(env) != 0;
   hipError_t e = hipMalloc(&counter, sizeof(unsigned));

@@ -80,7 +80,7 @@ struct MixedRunner {
   double* core_part = nullptr; // core pass: per-workgroup bin rows
   size_t core_cap = 0;
   int cus = 0;
-  int R = 4;                   // bins per lane (4 or 8)
+  int R = 4;                   // bins per lane (1, 2, 4 or 8): blocks of 64 R bins
   int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
   int forced_nseg = 0;         // > 0: fixed segments per block
   bool persistent = true;      // false: one workgroup per work item, so the hardware interleaves

@@ -892,14 +892,27 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
   const dim3 grid_blocks(plan.nblocks, plan.ntasks);
-  if (R == 4) {
-    k_far_eval<4><<<grid, 256, 0, s>>>(plan);
-    k_far_edges<4, false><<<grid_blocks, 256, 0, s>>>(plan);
-    k_far_edges<4, true><<<grid_blocks, 256, 0, s>>>(plan);
-  } else {
-    k_far_eval<8><<<grid, 256, 0, s>>>(plan);
-    k_far_edges<8, false><<<grid_blocks, 512, 0, s>>>(plan);
-    k_far_edges<8, true><<<grid_blocks, 512, 0, s>>>(plan);
+  switch (R) {
+    case 1:
+      k_far_eval<1><<<grid, 256, 0, s>>>(plan);
+      k_far_edges<1, false><<<grid_blocks, 64, 0, s>>>(plan);
+      k_far_edges<1, true><<<grid_blocks, 64, 0, s>>>(plan);
+      break;
+    case 2:
+      k_far_eval<2><<<grid, 256, 0, s>>>(plan);
+      k_far_edges<2, false><<<grid_blocks, 128, 0, s>>>(plan);
+      k_far_edges<2, true><<<grid_blocks, 128, 0, s>>>(plan);
+      break;
+    case 4:
+      k_far_eval<4><<<grid, 256, 0, s>>>(plan);
+      k_far_edges<4, false><<<grid_blocks, 256, 0, s>>>(plan);
+      k_far_edges<4, true><<<grid_blocks, 256, 0, s>>>(plan);
+      break;
+    default:
+      k_far_eval<8><<<grid, 256, 0, s>>>(plan);
+      k_far_edges<8, false><<<grid_blocks, 512, 0, s>>>(plan);
+      k_far_edges<8, true><<<grid_blocks, 512, 0, s>>>(plan);
+      break;
   }
   return hipGetLastError();
 }

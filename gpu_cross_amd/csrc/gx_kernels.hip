@@ -386,9 +386,10 @@ struct MixedShape {
   static constexpr int kBins = 64 * R;     // bins of one work item (one wave)
 };
 
-// Staged line record, 2 + R/4 float4 per line:
+// Staged line record, 2 + ceil(R/4) float4 per line:
 //   [0]      ib, ie, k0, k1 (ints: window and core ranges, absolute bins); anchored fast
-//            lines keep x at the middles of slices 0-3 here instead (they need no ranges)
+//            lines keep x at the middles of slices 0-3 (those of the R slices) here instead
+//            (they need no ranges)
 //   [1]      e, h, k, a      e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
 //   [2]      b, r, cf, s     b = a (scaled); r = x at the centre bin; cf: centre bin
 //                            relative to the wave's first bin; s: slice codes of an edge
@@ -397,7 +398,8 @@ struct MixedShape {
 // so the fast loop reads everything of a line from one LDS address.
 template <int R>
 struct StagedRec {
-  static constexpr int kVec = 2 + R / 4;
+  static constexpr int kGroups = (R + 3) / 4;   // float4 groups of slice anchors
+  static constexpr int kVec = 2 + kGroups;
 };
 
 // Anchored offsets: x at bin jb + o is A - o e, A = x(jb) rounded once from FP64 and o an
@@ -444,7 +446,7 @@ __device__ __forceinline__ void wave_lds_handover() {
 }
 
 template <int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ? 6 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ? 6 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
   constexpr int kBins = MixedShape<R>::kBins;
   constexpr int kWaves = kBlock / 64;
   constexpr int kVec = StagedRec<R>::kVec;
@@ -564,7 +566,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
           const double ed_ = gv.dfreq * g * sg;
           const double x32 = (nu - fc) * g * sg + (cj - (double)(w0 + 32)) * ed_;
 #pragma unroll
-          for (int qg = 0; qg < R / 4; ++qg) {
+          for (int qg = 0; qg < StagedRec<R>::kGroups; ++qg) {
             const double b0 = x32 - (double)(256 * qg) * ed_;
             rec[qg == 0 ? 0 : 2 + qg] = make_float4((float)b0, (float)(b0 - 64.0 * ed_),
                                                     (float)(b0 - 128.0 * ed_), (float)(b0 - 192.0 * ed_));
@@ -613,12 +615,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 4 ?
           const float4 E = my_l[k][1];
           const float bb = my_l[k][2].x;
 #pragma unroll
-          for (int qg = 0; qg < R / 4; ++qg) {
+          for (int qg = 0; qg < StagedRec<R>::kGroups; ++qg) {
             const float4 H = my_l[k][qg == 0 ? 0 : 2 + qg];
             const float hq[4] = {H.x, H.y, H.z, H.w};
 #pragma unroll
             for (int qq = 0; qq < 4; ++qq)
-              part[4 * qg + qq] += region1(__builtin_fmaf(no, E.x, hq[qq]), E.y, E.z, E.w, bb);
+              if (4 * qg + qq < R) part[4 * qg + qq] += region1(__builtin_fmaf(no, E.x, hq[qq]), E.y, E.z, E.w, bb);
           }
         }
       });
@@ -932,15 +934,17 @@ hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
-int mixed_bins_per_block(int R) { return R == 4 ? MixedShape<4>::kBins : MixedShape<8>::kBins; }
+int mixed_bins_per_block(int R) { return 64 * R; }
 
 hipError_t launch_accum_mixed(const MixedPlan& p, int R, hipStream_t s) {
   if (p.nitems <= 0) return hipSuccess;
   k_reset_counter<<<1, 1, 0, s>>>(p.counter);
-  if (R == 4)
-    k_accum_voigt_mixed<4><<<p.grid, kBlock, 0, s>>>(p);
-  else
-    k_accum_voigt_mixed<8><<<p.grid, kBlock, 0, s>>>(p);
+  switch (R) {
+    case 1: k_accum_voigt_mixed<1><<<p.grid, kBlock, 0, s>>>(p); break;
+    case 2: k_accum_voigt_mixed<2><<<p.grid, kBlock, 0, s>>>(p); break;
+    case 4: k_accum_voigt_mixed<4><<<p.grid, kBlock, 0, s>>>(p); break;
+    default: k_accum_voigt_mixed<8><<<p.grid, kBlock, 0, s>>>(p); break;
+  }
   hipError_t e = hipGetLastError();
   if (e) return e;
   if (p.nseg > 1) {
@@ -970,8 +974,13 @@ hipError_t launch_core_mixed(const MixedPlan& p, hipStream_t s) {
 
 int mixed_occupancy(int R) {
   int n = 0;
-  hipError_t e = R == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<4>, kBlock, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<8>, kBlock, 0);
+  hipError_t e;
+  switch (R) {
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<1>, kBlock, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<2>, kBlock, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<4>, kBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_accum_voigt_mixed<8>, kBlock, 0); break;
+  }
   if (e != hipSuccess) n = 1;
   return n > 0 ? n : 1;
 }

@@ -193,7 +193,7 @@ int autotune(gx_table* t) {
     int bR = 4, bI = 4;
     int last_nseg = -1, last_R = -1;
     int rc = GX_OK;
-    for (int R : {4, 8}) {
+    for (int R : {1, 2, 4, 8}) {
       for (int ips : {4, 8, 2}) {
         t->R = R;
         t->ips = ips;

@@ -172,6 +172,28 @@ def test_voigt_mixed_far_field(voigt_lines, monkeypatch, T, P):
     assert t_on.far_evals + t_on.core_evals < t_on.evals
 
 
+@pytest.mark.parametrize("R,T,P", [("1", 3000.0, 1e-3), ("1", 1000.0, 100.0), ("2", 300.0, 10.0)])
+def test_voigt_mixed_far_field_small_blocks(voigt_lines, monkeypatch, R, T, P):
+    """The far field with blocks of 64 and 128 bins (real and complex centres) against the
+    FP64 oracle, engaged."""
+    monkeypatch.setenv("GX_MIXED_R", R)
+    ll = voigt_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 5900.0) & (ll.nu < 6300.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(6000.0, 6200.0, 40000)
+    gam = L.gamma_at(P)
+    print(f"voigt mixed far field R={R} T={T} P={P} lines={L.n}", flush=True)
+    out, t = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                         cutoff=25.0)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=25.0)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(out, ref)
+    print(f"  far field: {t.far_evals / max(t.evals, 1):.1%} of pairs; max rel err {err:.3e}", flush=True)
+    assert err <= MIXED_TOL
+    assert t.far_evals > 0
+
+
 @pytest.mark.parametrize("T,cutoff", [(300.0, 0.3), (300.0, 1.2), (3000.0, 1.2), (3000.0, 0.3), (1000.0, 3.0)])
 def test_voigt_mixed_short_windows(voigt_lines, T, cutoff):
     # Windows of 120-1200 bins against 256-bin blocks: both window ends in one block, one end
@@ -275,10 +297,11 @@ def test_voigt_mixed_tiny_nu_lines():
     assert err <= MIXED_TOL
 
 
-@pytest.mark.parametrize("R", ["4", "8"])
+@pytest.mark.parametrize("R", ["1", "2", "4", "8"])
 def test_voigt_mixed_bins_per_lane_and_repeatability(voigt_lines, monkeypatch, R):
-    """Both launch shapes of the mixed kernel (4 and 8 bins per lane) against the oracle, and
-    the same inputs twice give the same grid bit for bit (fixed summation order)."""
+    """Every launch shape of the mixed kernel (1, 2, 4 and 8 bins per lane: blocks of 64 R
+    bins) against the oracle, and the same inputs twice give the same grid bit for bit
+    (fixed summation order)."""
     monkeypatch.setenv("GX_MIXED_R", R)
     ll = voigt_lines
     T, P = 800.0, 0.01
