@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -57,7 +58,7 @@ struct Launch {               // one persistent accumulation launch (a batch of 
 };
 
 // autotune cache: (bins, lines, points per launch) -> (R, items per slot)
-std::map<std::tuple<int, size_t, int>, std::pair<int, int>> g_tuned;
+std::map<std::tuple<int, size_t, int, int>, std::pair<int, int>> g_tuned;
 
 }  // namespace
 
@@ -153,9 +154,20 @@ int enqueue(gx_table* t, bool timed) {
   return GX_OK;
 }
 
+// GX_MIXED_R (1, 2, 4 or 8): a fixed block size of 64 R bins, else 0.
+int env_R() {
+  if (const char* env = std::getenv("GX_MIXED_R")) {
+    const int r = std::atoi(env);
+    if (r == 1 || r == 2 || r == 4 || r == 8) return r;
+  }
+  return 0;
+}
+
 int prepare_launches(gx_table* t) {
   for (Launch& l : t->launches) {
     const Group& g = t->groups[l.group];
+    int rc0 = l.run.init();                     // before R is set: init applies GX_MIXED_R to a runner's default
+    if (rc0) return rc0;
     l.run.R = t->R;
     l.run.items_per_slot = t->ips;
     std::vector<MixedTask> tasks(l.count);
@@ -180,7 +192,8 @@ int prepare_launches(gx_table* t) {
 // Time each launch shape on the first (up to 4) points of the first group and keep
 // the fastest; the choice is cached per (bins, lines, points per launch).
 int autotune(gx_table* t) {
-  const auto key = std::make_tuple(t->M, t->L->ls.n, t->launches.empty() ? 0 : t->launches[0].count);
+  const int forced_R = env_R();                 // GX_MIXED_R restricts the choice to that block size
+  const auto key = std::make_tuple(t->M, t->L->ls.n, t->launches.empty() ? 0 : t->launches[0].count, forced_R);
   auto it = g_tuned.find(key);
   if (it == g_tuned.end() && !t->launches.empty()) {
     std::vector<Launch> saved;
@@ -194,6 +207,7 @@ int autotune(gx_table* t) {
     int last_nseg = -1, last_R = -1;
     int rc = GX_OK;
     for (int R : {1, 2, 4, 8}) {
+      if (forced_R && R != forced_R) continue;
       for (int ips : {4, 8, 2}) {
         t->R = R;
         t->ips = ips;
@@ -434,6 +448,7 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
     for (Launch& l : t->launches)
       for (auto& ev : l.ev)
         if ((e = hipEventCreate(&ev))) return bail(hip_fail(e, "event"));
+    if (const int r = env_R()) t->R = r;
     if ((rc = o->autotune ? autotune(t) : prepare_launches(t))) return bail(rc);
   } else {
     int rc = t->der64.reserve(std::max<size_t>(n, 1));

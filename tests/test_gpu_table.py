@@ -67,6 +67,26 @@ def test_mixed_table_vs_oracle_and_graph_replays(setup):
     eager.close()
 
 
+@pytest.mark.parametrize("R,autotune", [("1", False), ("2", True), ("8", False)])
+def test_mixed_table_fixed_block_size(setup, monkeypatch, R, autotune):
+    """GX_MIXED_R fixes a table's block size (64 R bins) for its ranges, far field and
+    kernels alike; every grid against the oracle, with the far field engaged."""
+    monkeypatch.setenv("GX_MIXED_R", R)
+    ll, sub, lines, pts = setup
+    M = 30000
+    freq, d = make_grid(2000.0, 2200.0, M)
+    tab = Table(lines, 2000.0, 2200.0, M, pts, 17.03, cutoff=25.0, use_graph=False, autotune=autotune)
+    assert tab.shape()[0] == int(R)
+    tab.run()
+    t = tab.wait()
+    print(f"table R={R}: far field {t.far_evals / t.evals:.1%} of pairs", flush=True)
+    assert t.far_evals > 0
+    for p, (T, P, Q) in enumerate(pts):
+        err = rel_err(tab.read(p), oracle_grid(sub, T, P, Q, freq, d, 25.0))
+        assert err <= 1e-5, (p, err)
+    tab.close()
+
+
 def test_fp64_table_matches_oracle(setup):
     ll, sub, lines, pts = setup
     M = 20000
