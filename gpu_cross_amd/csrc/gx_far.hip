@@ -39,6 +39,9 @@
 
 #include "gx_internal.h"
 
+// Moments, series and direct sums here are approximations held to a truncation bound, so
+// they let the compiler fuse multiply-adds; the tests that decide which clusters serve which
+// bins (series_ok, serves, the plans) keep every rounding explicit.
 namespace gx {
 namespace {
 
@@ -129,7 +132,6 @@ __device__ __forceinline__ void wave_sums_f64(double (&v)[N], int lane) {
 // Leaf moments of every task: one wave per (leaf, task), one line per lane, the lanes'
 // terms summed by wave_sums_f64.
 __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
-#pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const int leaf = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (leaf >= P.far_n[0]) return;
@@ -225,7 +227,6 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
 // the same with a complex shift, M_k = sum_m C(k, m) a^(k-m) b^m M_m(child), formed by
 // repeated Taylor-shift passes v_k += a v_(k-1) over v_m = b^m M_m(child).
 __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
-#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.far_n[lev]) return;
   const size_t tbase = (size_t)blockIdx.y * P.far_nclus;
@@ -378,7 +379,6 @@ __device__ __forceinline__ bool serves(const double4 gm, const double* __restric
 
 // Sum of a cluster (record rc, centre nu_c) at bin f.
 __device__ __forceinline__ double cluster_sum(const double* __restrict__ rc, double nu_c, bool cplx, double f) {
-#pragma clang fp contract(off)
   if (!cplx) {
     const double id = 1.0 / (f - nu_c);
     const double t = rc[0] * id;
@@ -521,7 +521,6 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
 // Far-field sums of one (task, block) per wave, added to the task's grid.
 template <int R>
 __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
-#pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (b >= P.nblocks) return;
@@ -572,7 +571,6 @@ __device__ __forceinline__ int upper_bound_int(const int* __restrict__ v, int a,
 // Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
 // kernel's: c y (u + 1) / (u^2 + 2 y^2), u = x^2 + y^2 - 1/2, scaled by amp = S g).
 __device__ __forceinline__ double region1_line(double nu, double g, double y, float amp, double f) {
-#pragma clang fp contract(off)
   const double x = (nu_if(nu) - f) * g;
   const double y2 = y * y;
   const double u = x * x + y2 - 0.5;
@@ -580,7 +578,6 @@ __device__ __forceinline__ double region1_line(double nu, double g, double y, fl
 }
 
 __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedTask& tk, int a, int z, double f) {
-#pragma clang fp contract(off)
   double s = 0.0;
   int l = a;
   // four lines' loads issued together, then their terms added in line order
@@ -605,7 +602,6 @@ __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedT
 // from the largest aligned clusters that serve f, partial leaves line by line.
 __device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, bool cplx,
                             int a, int z, double f) {
-#pragma clang fp contract(off)
   if (z <= a) return 0.0;
   const int A = (a + kFarLeaf - 1) / kFarLeaf, B = z / kFarLeaf;
   if (A >= B) return region1_lines(P, tk, a, z, f);
@@ -655,7 +651,6 @@ struct RangeMoments {
 template <bool CPLX>
 __device__ __forceinline__ void leaf_to_range(const double* __restrict__ rc, double nu_l, double cE, double gE,
                                               double rE, double (&m)[RangeMoments<CPLX>::N]) {
-#pragma clang fp contract(off)
   const double b = rc[0] / rE;
   if (!CPLX) {
     const double a = (nu_l - cE) / rE;
@@ -705,7 +700,6 @@ __device__ __forceinline__ void leaf_to_range(const double* __restrict__ rc, dou
 template <bool CPLX>
 __device__ __forceinline__ double range_value(const double (&m)[RangeMoments<CPLX>::N], double cE, double gE,
                                               double rE, double f) {
-#pragma clang fp contract(off)
   if (!CPLX) {
     const double id = 1.0 / (f - cE);
     const double t = rE * id;
@@ -788,7 +782,6 @@ __device__ __forceinline__ void block_scan(double (&v)[N], double* s_tot) {
 template <int NW, bool CPLX, bool SUFFIX>
 __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, int lo,
                              int hi, double f, double f_lo, double f_hi, int* s_cut, double* s_tot, double* s_red) {
-#pragma clang fp contract(off)
   constexpr int NB = 64 * NW;
   const int tid = threadIdx.x;
   __syncthreads();                                             // s_cut of an earlier call read
@@ -833,7 +826,6 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
 // each range as at most one partial leaf line by line plus whole leaves (leaves_sum).
 template <int R, bool CPLX>
 __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
-#pragma clang fp contract(off)
   constexpr int NB = 64 * R;
   __shared__ int s_cut[NB];
   __shared__ double s_tot[R * RangeMoments<CPLX>::N];
