@@ -452,13 +452,62 @@ def bench_ingest(args, wl):
     print(json.dumps(res), flush=True)
 
 
+def bench_hitran(args, wl):
+    """HITRAN .par reader: 1e7 160-character records (1.6 GB of text), read for a 0-12000
+    cm^-1 grid with a 25 cm^-1 cutoff at 1 atm; a step reads the whole text (host threads,
+    the text in host memory)."""
+    from gpu_cross_amd import ingest
+
+    rng = np.random.default_rng(wl["seed"])
+    n = wl["n_lines"]
+    t0 = time.time()
+    nu = np.sort(rng.uniform(0.0, 12000.0, n))
+    aif = 10 ** rng.uniform(-3, 2, n)
+    aif[::50] = 0.0                                      # some records carry only the intensity
+    text = ingest.format_hitran(nu, 10 ** rng.uniform(-30, -19, n), aif, rng.uniform(0.01, 0.12, n),
+                                rng.uniform(0.05, 0.5, n), rng.uniform(0, 8000, n), rng.uniform(0.3, 0.8, n),
+                                rng.integers(1, 90, n).astype(float), rng.integers(1, 90, n).astype(float),
+                                mol_id=2, iso=1)
+    log(f"generated {n} records ({text.size / 1e9:.2f} GB) in {time.time() - t0:.1f} s")
+    start, end, _ = wl["grid"]
+    lo, hi = start - wl["cutoff"], end + wl["cutoff"]
+    walls = []
+    for i in range(max(args.warmup, 1) + args.steps):
+        t1 = time.perf_counter()
+        lines, mol, iso = ingest.read_hitran(text, pressure=1.0, mixture=0.8, ref_partition=286.09, lo=lo, hi=hi)
+        if i >= max(args.warmup, 1):
+            walls.append(time.perf_counter() - t1)
+            log(f"read step: {walls[-1] * 1e3:.1f} ms, {lines.records} records, {lines.n} kept")
+    wall = float(np.mean(walls))
+    threads = os.cpu_count()
+    res = {"metric": "HITRAN .par records read per second (host reader, text in host memory)",
+           "value": lines.records / wall, "unit": "records/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": f"hitran_1e7: {n} records ({text.size} bytes of .par text), range {lo}-{hi} cm^-1",
+                      "records": lines.records, "lines_kept": lines.n, "threads": threads},
+           "bytes_per_s": text.size / wall}
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+
+        probe = min(n, 2_000_000)
+        t0 = time.perf_counter()
+        _, rec, _, _ = po.read_hitran(bytes(text[:161 * probe]), 1.0, 0.8, 286.09, lo, hi)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": rec / dt, "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {rec} records of the same text ({dt:.1f} s), the oracle's "
+                                         "fgets/atof read loop, one thread as the reference"}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="table_48",
-                    choices=["table_48", "voigt_1e8", "voigt_1e7", "serving_3e8", "doppler_1e5", "ingest_1e8"])
+                    choices=["table_48", "voigt_1e8", "voigt_1e7", "serving_3e8", "doppler_1e5", "ingest_1e8",
+                             "hitran_1e7"])
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per load level")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -478,6 +527,8 @@ def main():
     from gpu_cross_amd.ingest import partition
     from gpu_cross_amd.synth import WORKLOADS, workload
 
+    if args.workload == "hitran_1e7":
+        return bench_hitran(args, dict(seed=11, n_lines=10_000_000, grid=(0.0, 12000.0, 1_200_000), cutoff=25.0))
     wl = WORKLOADS[args.workload]
     if args.workload == "table_48":
         return bench_table(args, wl)

@@ -14,7 +14,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgx.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gx.h")
-HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", h) for h in ("gx_ingest.h", "gx_output.h")]
+HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", h)
+                      for h in ("gx_ingest.h", "gx_output.h", "gx_hitran.h")]
 
 GX_OK, GX_INVALID_ARG, GX_OUT_OF_MEMORY, GX_NO_DEVICE, GX_DEVICE_ERROR, GX_BUSY, GX_NOT_FOUND = range(7)
 DOPPLER, VOIGT = 1, 2
@@ -47,6 +48,11 @@ class Timing(ctypes.Structure):
                 ("evals", ctypes.c_longlong), ("lines", ctypes.c_longlong), ("region1_ms", ctypes.c_float),
                 ("core_ms", ctypes.c_float), ("core_evals", ctypes.c_longlong), ("far_ms", ctypes.c_float),
                 ("far_evals", ctypes.c_longlong)]
+
+
+class HitranOpts(ctypes.Structure):
+    _fields_ = [("pressure", ctypes.c_double), ("mixture", ctypes.c_double), ("ref_partition", ctypes.c_double),
+                ("lo", ctypes.c_double), ("hi", ctypes.c_double), ("nthreads", ctypes.c_int)]
 
 
 class Grid(ctypes.Structure):
@@ -171,6 +177,10 @@ def load():
     _sig(L, "gx_ingest_clear", ctypes.c_int, [P])
     _sig(L, "gx_ingest_last_feed", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float), szp, szp])
     _sig(L, "gx_format_trans", ctypes.c_int, [sz, _ip, _ip, _dp, P, sz, szp])
+    _sig(L, "gx_hitran_read", ctypes.c_int, [P, sz, ctypes.POINTER(HitranOpts), sz, _dp, _dp, _dp, _ip, _dp, _dp,
+                                             szp, szp, _ip, _ip])
+    _sig(L, "gx_format_hitran", ctypes.c_int, [sz, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                               _dp, P, sz, szp])
     _sig(L, "gx_xsec_text", ctypes.c_int, [_dp, _dp, ctypes.c_int, P, sz, szp])
     _sig(L, "gx_xsec_text_write", ctypes.c_int, [ctypes.c_char_p, _dp, _dp, ctypes.c_int])
     _sig(L, "gx_opacity_write", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(OpacityInfo), ctypes.POINTER(Point),

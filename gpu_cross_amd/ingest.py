@@ -237,3 +237,41 @@ def format_trans(id_f: np.ndarray, id_i: np.ndarray, aif: np.ndarray) -> np.ndar
     capi.check(capi.load().gx_format_trans(f.size, capi.iptr(f), capi.iptr(i), capi.dptr(a), buf.ctypes.data, cap,
                                            ctypes.byref(wrote)))
     return buf[:wrote.value]
+
+
+def read_hitran(src: Union[str, bytes, os.PathLike, np.ndarray], pressure: float = 1.0, mixture: float = 1.0,
+                ref_partition: float = 0.0, lo: float = -np.inf, hi: float = np.inf, nthreads: int = 0):
+    """HITRAN .par text read as HITRANStateReader and the Voigt driver read it
+    (HITRANStateReader.cpp:3-250, VoigtProfile.cpp:61-75): (Lines, molecule id, isotope).
+    ref_partition = Q(296 K), used for records whose A field is 0."""
+    t = _text(src if isinstance(src, np.ndarray) else _read_bytes(src))
+    L = capi.load()
+    o = capi.HitranOpts(float(pressure), float(mixture), float(ref_partition), float(lo), float(hi), int(nthreads))
+    n, rec = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    mol, iso = ctypes.c_int(0), ctypes.c_int(0)
+    # 161 bytes per HITRAN record; a text with shorter records reports its count and is read again
+    cap = t.size // 161 + 16
+    while True:
+        out = [np.empty(cap), np.empty(cap), np.empty(cap), np.empty(cap, np.int32), np.empty(cap), np.empty(cap)]
+        rc = L.gx_hitran_read(t.ctypes.data, t.size, ctypes.byref(o), cap, capi.dptr(out[0]), capi.dptr(out[1]),
+                              capi.dptr(out[2]), capi.iptr(out[3]), capi.dptr(out[4]), capi.dptr(out[5]),
+                              ctypes.byref(n), ctypes.byref(rec), ctypes.byref(mol), ctypes.byref(iso))
+        if rc == capi.GX_INVALID_ARG and n.value > cap:
+            cap = n.value
+            continue
+        capi.check(rc)
+        break
+    m = n.value
+    return Lines(*[a[:m] for a in out], rec.value), mol.value, iso.value
+
+
+def format_hitran(nu, intens, aif, gamma_air, gamma_self, elow, nexp, gup, glow, mol_id: int = 11,
+                  iso: int = 1) -> np.ndarray:
+    """HITRAN .par text (uint8 array), one 160-character record per line."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (nu, intens, aif, gamma_air, gamma_self, elow, nexp, gup, glow)]
+    n = arrs[0].size
+    buf = np.empty(161 * n + 1, np.uint8)
+    wrote = ctypes.c_size_t(0)
+    capi.check(capi.load().gx_format_hitran(n, mol_id, iso, *[capi.dptr(a) for a in arrs], buf.ctypes.data, buf.size,
+                                            ctypes.byref(wrote)))
+    return buf[:wrote.value]

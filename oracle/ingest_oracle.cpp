@@ -10,6 +10,12 @@
  *                       after clear(): "a1"JiJf"a0"Ji"-1"; gamma *= pressure)
  *   ReadNextState       ExomolStateReader.cpp:130-167 (stream >> id_f >> id_i >> aif)
  *   the read loop       VoigtProfile.cpp:56-68 (continue below lo, break above hi, per file)
+ * and the HITRAN .par reader:
+ *   GetField            HITRANStateReader.cpp:3-40  (fixed widths, 13-byte zeroed copy cut
+ *                       at 12, atof / atoi)
+ *   ReadNextState       HITRANStateReader.cpp:136-250 (fgets 1024 into one buffer kept for
+ *                       the file; gamma mixing, A from the intensity when A == 0)
+ *   the read loop       VoigtProfile.cpp:61-75 (the same filter; g stored as int)
  * gamma / n before the first key match are 0 here (uninitialised in the reference,
  * VoigtProfile.cpp:41); a blank .broad line is skipped (the reference indexes an empty
  * split, undefined).
@@ -138,6 +144,74 @@ long orc_ingest(const double* energy, const int* gns, const int* J, long nstates
       ++kept;
     }
   }
+  *records = nrec;
+  return kept;
+}
+
+/* HITRANStateReader over one .par text in memory: the constructor's first record gives the
+ * molecule and isotope, then ReadNextState record by record with the driver's filter.  The
+ * reader's buffer starts zeroed (uninitialised in the reference). */
+long orc_hitran(const char* text, long len, double pressure, double mixture, double ref_partition, double lo,
+                double hi, long cap, double* elow, double* nu, double* aif, int* gns, double* gamma, double* nexp,
+                long* records, int* mol_id, int* iso) {
+  static const int width[19] = {2, 1, 12, 10, 10, 5, 5, 10, 4, 8, 15, 15, 15, 15, 6, 12, 1, 7, 7};
+  char buffer[1024];
+  std::memset(buffer, 0, sizeof buffer);
+  auto copy_field = [&](int f, char* c) {
+    int at = 0;
+    for (int i = 0; i < f; ++i) at += width[i];
+    std::memset(c, 0, 13);
+    std::memcpy(c, buffer + at, (size_t)width[f]);
+    c[12] = '\0';
+  };
+  auto as_double = [&](int f) {
+    char c[13];
+    copy_field(f, c);
+    return std::atof(c);
+  };
+  auto as_int = [&](int f) {
+    char c[13];
+    copy_field(f, c);
+    return std::atoi(c);
+  };
+  *records = 0;
+  *mol_id = *iso = 0;
+  if (len <= 0) return 0;
+  FILE* fp = fmemopen(const_cast<char*>(text), (size_t)len, "r");
+  if (!fp) return -1;
+  if (std::fgets(buffer, 1024, fp)) {
+    *mol_id = as_int(0);
+    *iso = as_int(1);
+  }
+  std::rewind(fp);
+  const double planck = 6.6260693e-27, vellgt = 2.99792458e10, boltz = 1.380658e-16;
+  const double pi = std::acos(-1.0);
+  long kept = 0, nrec = 0;
+  while (std::fgets(buffer, 1024, fp)) {
+    ++nrec;
+    double v = as_double(2), e = as_double(7), a = as_double(4);
+    const double gair = as_double(5), gself = as_double(6), n = as_double(8);
+    double g = as_double(17);
+    const double s = as_double(3);
+    const double gam = (gair * mixture + gself * (1.0 - mixture)) * pressure;
+    if (a == 0.0) {
+      const double beta = planck * vellgt / (boltz * 296.0);
+      a = s * 8.0 * pi * vellgt * ref_partition * v * v / (std::exp(-beta * e) * (1.0 - std::exp(-beta * v)));
+      g = 1.0;
+    }
+    if (v < lo) continue;
+    if (v > hi) break;
+    if (kept < cap) {
+      elow[kept] = e;
+      nu[kept] = v;
+      aif[kept] = a;
+      gns[kept] = (int)g;
+      gamma[kept] = gam;
+      nexp[kept] = n;
+    }
+    ++kept;
+  }
+  std::fclose(fp);
   *records = nrec;
   return kept;
 }

@@ -214,6 +214,7 @@ def ingest_lib():
         L = ctypes.CDLL(path)
         L.orc_states.restype = ctypes.c_long
         L.orc_ingest.restype = ctypes.c_long
+        L.orc_hitran.restype = ctypes.c_long
         _ingest_lib = L
     return _ingest_lib
 
@@ -250,3 +251,22 @@ def read_trans(states: dict, broad: Optional[bytes], pressure: float, lo: float,
     if n < 0:
         raise ValueError(f"oracle reader: state id out of range ({n})")
     return [a[:n] for a in arrs], rec.value
+
+
+def read_hitran(text: bytes, pressure: float, mixture: float, ref_partition: float, lo: float, hi: float):
+    """HITRANStateReader's read of one .par text with the Voigt driver's filter
+    (HITRANStateReader.cpp:3-250, VoigtProfile.cpp:61-75).  Returns ([E'', nu, A, g, gamma, n],
+    records read, molecule id, isotope)."""
+    L = ingest_lib()
+    cap = bytes(text).count(b"\n") + len(text) // 1023 + 2      # fgets pieces at most
+    arrs = [np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap), np.zeros(cap)]
+    rec, mol, iso = ctypes.c_long(0), ctypes.c_int(0), ctypes.c_int(0)
+    n = L.orc_hitran(bytes(text), ctypes.c_long(len(text)), ctypes.c_double(pressure), ctypes.c_double(mixture),
+                     ctypes.c_double(ref_partition), ctypes.c_double(lo), ctypes.c_double(hi), ctypes.c_long(cap),
+                     _d(arrs[0]), _d(arrs[1]), _d(arrs[2]), _i(arrs[3]), _d(arrs[4]), _d(arrs[5]), ctypes.byref(rec),
+                     ctypes.byref(mol), ctypes.byref(iso))
+    if n < 0:
+        raise OSError("oracle HITRAN reader: fmemopen failed")
+    if n > cap:
+        raise ValueError("oracle HITRAN reader: more lines than the capacity")
+    return [a[:n] for a in arrs], rec.value, mol.value, iso.value

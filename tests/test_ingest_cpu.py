@@ -176,3 +176,113 @@ def test_decimal_parser_against_stream_and_strtod():
     # the ExoMol columns are always converted on the device
     exo = [l for l in r.stdout.splitlines() if l.startswith("exomol")][0].split()
     assert exo[exo.index("back") + 1] == "0"
+
+
+# --- HITRAN .par reader ---------------------------------------------------------------
+# One 160-character record laid out column by column as HITRAN's format fixes it (molecule
+# I2, isotope I1, nu F12.6, S E10.3, A E10.3, gamma_air F5.4, gamma_self F5.3, E'' F10.4,
+# n F4.2, delta F8.6, four 15-character quanta, error 6, references 12, flag 1, g' F7.1,
+# g'' F7.1), written out by hand so that the expected values below do not come from the
+# code under test.
+def _hitran_record(mol="12", iso="1", nu="  1234.567890", s=" 1.234E-20", a=" 5.678E+01", gair=".0712",
+                   gself="0.345", elow="  123.4567", n="0.75", delta="-.001234", gup="   27.0", glow="   25.0"):
+    rec = (mol + iso + nu[-12:] + s + a + gair + gself + elow + n + delta + " " * 60 + " " * 6 + " " * 12 + " "
+           + gup + glow)
+    assert len(rec) == 160
+    return rec + "\n"
+
+
+HITRAN_PLANCK, HITRAN_VELLGT, HITRAN_BOLTZ = 6.6260693e-27, 2.99792458e10, 1.380658e-16
+
+
+def test_oracle_hitran_known_answer():
+    """Fields at their columns, gamma = (gamma_air x + gamma_self (1 - x)) P, g' stored as int,
+    molecule and isotope from the first record (HITRANStateReader.cpp:3-58, 207-247)."""
+    text = (_hitran_record() + _hitran_record(nu="   250.000000", gup="   10.9")).encode()
+    (e, nu, a, g, gam, n), rec, mol, iso = po.read_hitran(text, 2.0, 0.7, 100.0, 0.0, 1e9)
+    assert (mol, iso, rec) == (12, 1, 2)
+    assert list(nu) == [1234.56789, 250.0]
+    assert list(e) == [123.4567, 123.4567] and list(a) == [56.78, 56.78] and list(n) == [0.75, 0.75]
+    assert list(g) == [27, 10]                     # 10.9 truncated by the int store
+    assert gam[0] == (0.0712 * 0.7 + 0.345 * (1.0 - 0.7)) * 2.0
+
+
+def test_oracle_hitran_einstein_a_from_intensity():
+    """A record with A == 0 gets A from its intensity and Q(296 K), with g = 1: fed back into
+    the path's intensity formula at T = 296 K and Q = Q(296 K), it returns the HITRAN
+    intensity times the path's ISQRTPI factor (the two formulas are inverses)."""
+    text = _hitran_record(a=" 0.000E+00", s=" 3.250E-21", nu="  2100.250000", elow=" 1500.0000").encode()
+    (e, nu, a, g, gam, n), rec, mol, iso = po.read_hitran(text, 1.0, 1.0, 286.09, 0.0, 1e9)
+    assert g[0] == 1
+    beta = HITRAN_PLANCK * HITRAN_VELLGT / (HITRAN_BOLTZ * 296.0)
+    cmcoef = 1.0 / (8.0 * math.pi * HITRAN_VELLGT)
+    S = cmcoef * a[0] * g[0] * math.exp(-beta * e[0]) * (1.0 - math.exp(-beta * nu[0])) / (nu[0] * nu[0] * 286.09)
+    assert abs(S / 3.25e-21 - 1.0) < 1e-14
+
+
+def test_oracle_hitran_short_record_reads_the_previous_buffer():
+    """The reader's buffer is kept for the whole file: a record too short for the fields it
+    reads takes the missing bytes from the record before (its fgets left them there); the
+    driver's filter skips below lo and stops at the first line above hi."""
+    long = _hitran_record()
+    short = long[:50] + "\n"                       # ends inside gamma_self: E'' onwards are stale
+    text = (long + short + _hitran_record(nu="  3000.000000") + _hitran_record(nu="  1300.000000")).encode()
+    (e, nu, a, g, gam, n), rec, mol, iso = po.read_hitran(text, 1.0, 1.0, 100.0, 1000.0, 2000.0)
+    assert rec == 3                                 # the record above hi ends the read
+    assert list(nu) == [1234.56789, 1234.56789]
+    # the short record (51 characters with its line feed): gamma_air and gamma_self (columns
+    # 35-44) are its own; E'' (columns 45-54) holds "  123", its line feed and terminator, so
+    # atof reads 123; n (55-58) and g' (146-152) are the previous record's bytes
+    assert gam[1] == 0.0712
+    assert e[1] == 123.0 and n[1] == 0.75 and g[1] == 27
+
+
+@pytest.fixture(scope="module")
+def hitran_text():
+    rng = np.random.default_rng(5)
+    n = 4000
+    nu = np.sort(rng.uniform(10.0, 9000.0, n))
+    A = 10 ** rng.uniform(-3, 2, n)
+    A[::9] = 0.0
+    args = (nu, 10 ** rng.uniform(-30, -19, n), A, rng.uniform(0.01, 0.12, n), rng.uniform(0.05, 0.5, n),
+            rng.uniform(0, 8000, n), rng.uniform(0.3, 0.8, n), rng.integers(1, 90, n).astype(float),
+            rng.integers(1, 90, n).astype(float))
+    text = bytes(ingest.format_hitran(*args, mol_id=2, iso=1))
+    # odd records: a short one, one longer than fgets' 1023 characters, no final newline
+    lines = text.split(b"\n")
+    lines[100] = lines[100][:70]
+    lines[200] = lines[200] + b" " * 1200
+    return b"\n".join(lines[:-1])
+
+
+@pytest.mark.parametrize("nthreads,lo,hi", [(1, -1e9, 1e9), (3, 500.0, 7000.0), (0, 2500.0, 2600.0)])
+def test_product_hitran_reader_matches_oracle(hitran_text, nthreads, lo, hi):
+    L, mol, iso = ingest.read_hitran(hitran_text, pressure=3.0, mixture=0.6, ref_partition=286.09, lo=lo, hi=hi,
+                                     nthreads=nthreads)
+    ref, rec, m2, i2 = po.read_hitran(hitran_text, 3.0, 0.6, 286.09, lo, hi)
+    assert (L.records, mol, iso) == (rec, m2, i2)
+    # a blank piece (the tail of the overlong line) reads nu = 0 and A = 0, and the
+    # reference's A from the intensity is then 0/0: NaN on both sides
+    for got, exp in zip([L.elow, L.nu, L.aif, L.gns, L.gamma, L.nexp], ref):
+        assert np.array_equal(got, exp, equal_nan=True)
+
+
+def test_product_hitran_reader_empty_and_format():
+    L, mol, iso = ingest.read_hitran(b"")
+    assert L.n == 0 and L.records == 0 and (mol, iso) == (0, 0)
+    rec = bytes(ingest.format_hitran([1234.56789], [1.234e-20], [56.78], [0.0712], [0.345], [123.4567], [0.75],
+                                     [27.0], [25.0], mol_id=12, iso=1))
+    exp = _hitran_record(delta="0.000000")
+    assert rec.decode() == exp
+
+
+def test_product_hitran_reader_short_records():
+    """Records far shorter than 160 characters (more records than the reader's first guess
+    of the count) read as the oracle reads them."""
+    text = b"".join(b"02%13.6f%10.3E\n" % (100.0 + 0.5 * i, 1e-21 * (i % 7)) for i in range(5000))
+    L, mol, iso = ingest.read_hitran(text, ref_partition=300.0, lo=-1e9, hi=1e9)
+    ref, rec, m2, i2 = po.read_hitran(text, 1.0, 1.0, 300.0, -1e9, 1e9)
+    assert L.n == 5000 and (L.records, mol, iso) == (rec, m2, i2)
+    for got, exp in zip([L.elow, L.nu, L.aif, L.gns, L.gamma, L.nexp], ref):
+        assert np.array_equal(got, exp, equal_nan=True)
+
