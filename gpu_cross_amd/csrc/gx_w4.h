@@ -35,29 +35,28 @@ __device__ __forceinline__ double w4_region1(double x, double y) {
 }
 
 // The region tests of W4, each operand rounded as the reference's plain C rounds it (no
-// multiply-add fused across them): x + y against 15 and 5.5, y against 0.195 x - 0.176.
-struct W4Region {
-  bool r1, r2, r3;
-};
-
-__device__ __forceinline__ W4Region w4_region(double x, double y) {
+// multiply-add fused into them): x + y against 15 and 5.5, y against 0.195 x - 0.176.
+__device__ __forceinline__ double w4_sum(double x, double y) {   // x + y: regions I and II
 #pragma clang fp contract(off)
-  const double s = x + y;
+  return x + y;
+}
+
+__device__ __forceinline__ bool w4_region3(double x, double y) {   // region III (given not II)
+#pragma clang fp contract(off)
   const double b = 0.195 * x - 0.176;
-  return {s >= 15.0, s >= 5.5, y >= b};
+  return y >= b;
 }
 
 // Regions II-IV, FP64 (x >= 0 assumed by the callers; only |x| enters).
 __device__ __noinline__ double w4_inner(double x, double y) {
   const cplx t{y, -x};
-  const W4Region rg = w4_region(x, y);
-  if (rg.r2) {  // region II
+  if (w4_sum(x, y) >= 5.5) {  // region II
     const cplx u = cmul(t, t);
     const cplx num = cmul(t, cadd(1.410474, cscale(u, 0.5641896)));
     const cplx den = cadd(0.75, cmul(u, cadd(3.0, u)));
     return cdiv_re(num, den);
   }
-  if (rg.r3) {  // region III
+  if (w4_region3(x, y)) {  // region III
     cplx n = cscale(t, 0.5642236);
     n = cmul(t, cadd(3.778987, n));
     n = cmul(t, cadd(11.96482, n));
@@ -93,7 +92,7 @@ __device__ __noinline__ double w4_inner(double x, double y) {
 
 // Full W4, FP64.  x >= 0.
 __device__ __forceinline__ double w4(double x, double y) {
-  if (w4_region(x, y).r1) return w4_region1(x, y);
+  if (w4_sum(x, y) >= 15.0) return w4_region1(x, y);
   return w4_inner(x, y);
 }
 
@@ -108,9 +107,8 @@ __device__ __forceinline__ double w4(double x, double y) {
 // takes exp(u) = e^(y^2-x^2)(cos 2xy - i sin 2xy) from the FP32 hardware
 // exp/cos (2.7e-6 max).  The region is decided in FP64 by the caller's x, y.
 __device__ __forceinline__ float w4_core_mixed(double xd, double yd) {
-  const W4Region rg = w4_region(xd, yd);
   const float x = (float)xd, y = (float)yd;
-  if (rg.r2) {  // region II: t (1.410474 + 0.5641896 u) / (0.75 + u (3 + u))
+  if (w4_sum(xd, yd) >= 5.5) {  // region II: t (1.410474 + 0.5641896 u) / (0.75 + u (3 + u))
     const float ur = y * y - x * x, ui = -2.0f * x * y;
     const float ar = 1.410474f + 0.5641896f * ur, ai = 0.5641896f * ui;
     const float nr = y * ar + x * ai, ni = y * ai - x * ar;            // t * a, t = y - i x
@@ -118,7 +116,7 @@ __device__ __forceinline__ float w4_core_mixed(double xd, double yd) {
     const float dr = 0.75f + (ur * br - ui * bi), di = ur * bi + ui * br;
     return (nr * dr + ni * di) * __builtin_amdgcn_rcpf(dr * dr + di * di);
   }
-  if (rg.r3) {  // region III
+  if (w4_region3(xd, yd)) {  // region III
     const float tr = y, ti = -x;
     float nr = tr * 0.5642236f, ni = ti * 0.5642236f;
     float ar, ai;
