@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compensated add
 # HBM bytes per accumulation launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # same command scaled by a plain-copy calibration (profiles/k_accum_voigt_mixed.table_48.hbm.md)
-PROFILED_TRAFFIC = {"table_48": 2.377e11}
+PROFILED_TRAFFIC = {"table_48": 2.903e11}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
