@@ -110,19 +110,24 @@ __device__ __forceinline__ double wave_extreme(double v) {
 // Afterwards every lane holds the totals of values first(lane) + [0, N/8), first(lane) =
 // N/2 bit5 + N/4 bit4 + N/8 bit3 of its lane index.  Fixed order (a + b = b + a in IEEE):
 // the totals do not depend on scheduling.
-template <int N>
-__device__ __forceinline__ void wave_sums_f64(double (&v)[N], int lane) {
+template <int D, int H, int N, class T>
+__device__ __forceinline__ void wave_halve(T (&v)[N], int lane) {
 #pragma clang fp contract(off)
+  const bool up = (lane & D) != 0;
 #pragma unroll
-  for (int d = 32, h = N / 2; d >= 8; d >>= 1, h >>= 1) {
-    const bool up = (lane & d) != 0;
-#pragma unroll
-    for (int i = 0; i < h; ++i) {
-      const double keep = up ? v[h + i] : v[i];
-      const double send = up ? v[i] : v[h + i];
-      v[i] = keep + __shfl_xor(send, d);
-    }
+  for (int i = 0; i < H; ++i) {
+    const T keep = up ? v[H + i] : v[i];
+    const T send = up ? v[i] : v[H + i];
+    v[i] = keep + __shfl_xor(send, D);
   }
+}
+
+template <int N, class T>
+__device__ __forceinline__ void wave_sums(T (&v)[N], int lane) {
+#pragma clang fp contract(off)
+  wave_halve<32, N / 2>(v, lane);
+  wave_halve<16, N / 4>(v, lane);
+  wave_halve<8, N / 8>(v, lane);
 #pragma unroll
   for (int d = 4; d >= 1; d >>= 1)
 #pragma unroll
@@ -130,7 +135,12 @@ __device__ __forceinline__ void wave_sums_f64(double (&v)[N], int lane) {
 }
 
 // Leaf moments of every task: one wave per (leaf, task), one line per lane, the lanes'
-// terms summed by wave_sums_f64.
+// terms summed by wave_sums.  The radius, centre and the extremes the truncation bounds
+// use are FP64; each line's scaled pole offsets w (|w| <= 1) and its weight go to FP32, and
+// the powers and the wave sums run in FP32: a moment's rounding error is then ~1e-7 of the
+// sum of its terms' magnitudes, and the series weighs moment k by at most eta^(k+1), so the
+// far sums keep ~1e-7 relative accuracy (the weights are 0.28 S, S/alpha being FP32 in the
+// mixed path already).  The records stay FP64 for the tree and the plans.
 __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   const int lane = threadIdx.x & 63;
   const int leaf = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -164,8 +174,8 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   double r = sqrt(r2);
   if (!(r > 0.0)) r = 1.0;
   const double ir = 1.0 / r;
-  const double kap = act ? kKappa * (double)amp * al : 0.0;   // amp = S g
-  const double wi = wim * ir;
+  const float kap = act ? (float)(kKappa * (double)amp * al) : 0.0f;   // amp = S g
+  const float wi = (float)(wim * ir);
   double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
   if (lane == 0) {
     rec[0] = r;
@@ -177,47 +187,47 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   }
   // every lane's terms for all moments, then one halving reduction over the wave; lanes
   // 8m (m < 8) write the totals they hold
-  const double wrp = dp * ir, wrm = dm * ir;
+  const float wrp = (float)(dp * ir), wrm = (float)(dm * ir);
   const int first = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
   if (!cplx) {
     // Im w^k of each pole by the real recurrence z_k = 2 Re(w) z_(k-1) - |w|^2 z_(k-2)
     // (w and its conjugate are the roots of t^2 - 2 Re(w) t + |w|^2), z_0 = 0, z_1 = Im w
-    double v[kFarP];
-    const double tp = 2.0 * wrp, np = wrp * wrp + wi * wi;
-    const double tm = 2.0 * wrm, nm = wrm * wrm + wi * wi;
-    double p1 = wi, p0 = 0.0, m1 = wi, m0 = 0.0;
+    float v[kFarP];
+    const float tp = 2.0f * wrp, np = wrp * wrp + wi * wi;
+    const float tm = 2.0f * wrm, nm = wrm * wrm + wi * wi;
+    float p1 = wi, p0 = 0.0f, m1 = wi, m0 = 0.0f;
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) {
-      v[k] = kap * p1 + kap * m1;
-      const double p2 = tp * p1 - np * p0, m2 = tm * m1 - nm * m0;
+      v[k] = kap * (p1 + m1);
+      const float p2 = tp * p1 - np * p0, m2 = tm * m1 - nm * m0;
       p0 = p1;
       p1 = p2;
       m0 = m1;
       m1 = m2;
     }
-    wave_sums_f64(v, lane);
+    wave_sums(v, lane);
     if ((lane & 7) == 0) {
 #pragma unroll
-      for (int t = 0; t < kFarP / 8; ++t) rec[kFarHead + first * (kFarP / 8) + t] = v[t];
+      for (int t = 0; t < kFarP / 8; ++t) rec[kFarHead + first * (kFarP / 8) + t] = (double)v[t];
     }
   } else {
-    double v[2 * kFarP];                                       // (Re, Im) of M_0..M_(P-1)
-    double pr = 1.0, pi = 0.0, mr = 1.0, mi = 0.0;             // w+^0, w-^0
+    float v[2 * kFarP];                                        // (Re, Im) of M_0..M_(P-1)
+    float pr = 1.0f, pi = 0.0f, mr = 1.0f, mi = 0.0f;          // w+^0, w-^0
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) {
-      v[2 * k] = kap * pr + kap * mr;
-      v[2 * k + 1] = kap * pi + kap * mi;
-      const double nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
-      const double n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
+      v[2 * k] = kap * (pr + mr);
+      v[2 * k + 1] = kap * (pi + mi);
+      const float nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
+      const float n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
       pr = nr;
       pi = ni;
       mr = n2r;
       mi = n2i;
     }
-    wave_sums_f64(v, lane);
+    wave_sums(v, lane);
     if ((lane & 7) == 0) {
 #pragma unroll
-      for (int t = 0; t < kFarP / 4; ++t) rec[kFarHead + first * (kFarP / 4) + t] = v[t];
+      for (int t = 0; t < kFarP / 4; ++t) rec[kFarHead + first * (kFarP / 4) + t] = (double)v[t];
     }
   }
 }
