@@ -728,17 +728,6 @@ __global__ void k_centre_starts(GridView gv, const double* __restrict__ nu, int 
   starts[c] = lo;
 }
 
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_take(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
-}
-
-// row_shr within rows of 16, 0 where the source lane is outside the row
-template <int CTRL>
-__device__ __forceinline__ float dpp_shr(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
-}
-
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -751,10 +740,39 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// kFreqLds: grid values of the row staged in LDS next to the sums (else read from HBM).
+// Segmented inclusive sum over the wave in a fixed order: v += m * v(lane - 1, 2, 4, 8) inside
+// rows of 16 (row_shr, 0 from outside the row), then += m15 * v(lane 15 of the row before) on
+// rows 1 and 3 and += m31 * v(lane 31) on rows 2 and 3 (row_bcast; the other rows are left
+// as they are).  The m are 0 or 1 per lane (is the source lane in my segment), so each
+// segment's last lane ends with its total.  Each step is one DPP-modified FMA; the s_nops
+// give the two wait states a DPP read of a just-written VGPR needs (five before the first,
+// which also covers a VALU write of EXEC).  Every lane of the wave must be active.
+__device__ __forceinline__ float seg_scan(float v, float m1, float m2, float m4, float m8, float m15, float m31) {
+  asm volatile(
+      "s_nop 4\n\t"
+      "v_fmac_f32_dpp %0, %0, %1 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %2 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %3 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %4 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %5 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %0, %6 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(v)
+      : "v"(m1), "v"(m2), "v"(m4), "v"(m8), "v"(m15), "v"(m31));
+  return v;
+}
+
+// kFreqLds: grid values of the row staged in LDS, interleaved with the sums (entry i of the
+// row at s_core[2i], its frequency at s_core[2i + 1]); else the sums alone and the
+// frequencies read from HBM.
 template <bool kFreqLds>
 __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
   extern __shared__ double s_core[];
+  constexpr int kStride = kFreqLds ? 2 : 1;
   const int lane = threadIdx.x;
   const int w = blockIdx.x;
   const MixedTask tk = P.tasks[blockIdx.y];
@@ -762,15 +780,12 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
   const int span = P.span;
   const int c0 = P.cbase + w * P.cpw;
   const int base = c0 - P.kmax;                 // row entry i is bin base + i
-  double* acc = s_core;
-  const double* fq = kFreqLds ? s_core + span : gv.freq;
-  const int fq_off = kFreqLds ? base : 0;
   const int s0 = gv.start_idx, s1 = gv.start_idx + gv.points;
   for (int i = lane; i < span; i += 64) {
-    acc[i] = 0.0;
+    s_core[kStride * i] = 0.0;
     if (kFreqLds) {
       const int jj = base + i;
-      s_core[span + i] = (jj >= s0 && jj < s1) ? gv.freq[jj] : 0.0;
+      s_core[2 * i + 1] = (jj >= s0 && jj < s1) ? gv.freq[jj] : 0.0;
     }
   }
   __syncthreads();
@@ -794,10 +809,13 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
     }
     const bool has = act && core.y > core.x;
     if (!__ballot(has)) continue;
-    const int kmin = max(wave_min(has ? core.x - cj : 0x3fffffff), -P.kmax);
-    const int kend = min(wave_max(has ? core.y - cj : -0x3fffffff), P.kmax + 1);
+    // offsets k (bin cj + k) of this lane's core inside the row: [lo, hi)
+    const int lo = has ? max(core.x, base) - cj : 0x3fffffff;
+    const int hi = has ? min(core.y, base + span) - cj : -0x3fffffff;
+    const int kmin = __builtin_amdgcn_readfirstlane(max(wave_min(lo), -P.kmax));
+    const int kend = __builtin_amdgcn_readfirstlane(min(wave_max(hi), P.kmax + 1));
     // segments: lanes with equal centre bins (contiguous, nu is sorted); each lane's distance
-    // from its segment's first lane fixes which steps of the scan below it takes
+    // from its segment's first lane fixes which steps of the scan it takes
     const int cprev = __shfl_up(cj, 1), cnext = __shfl_down(cj, 1);
     const bool head = lane == 0 || cprev != cj;
     const bool tail = lane == 63 || cnext != cj;
@@ -807,29 +825,25 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
     const float m8 = hd >= 8 ? 1.0f : 0.0f;
     const float m15 = hd >= (lane & 15) + 1 ? 1.0f : 0.0f;   // lane 16 row - 1 is in my segment
     const float m31 = hd >= lane - 31 ? 1.0f : 0.0f;          // lane 31 is in my segment
+    // the lane's row entries for offset 0 (a segment's lanes share them; inactive lanes
+    // never use theirs)
+    const int rel = act ? cj - base : 0;
+    double* const acc0 = s_core + kStride * rel;
+    const double* const fq0 = kFreqLds ? acc0 + 1 : gv.freq + (base + rel);
     for (int k = kmin; k < kend; ++k) {
-      const int j = cj + k;
       float v = 0.0f;
-      const bool on = has && j >= core.x && j < core.y && (unsigned)(j - base) < (unsigned)span;
-      if (on) {
-        const double x = fabs(nu - fq[j - fq_off]) * g;
+      if (k >= lo && k < hi) {
+        const double x = fabs(nu - fq0[kStride * k]) * g;
         v = amp * w4_core_mixed(x, y);
       }
-      // segmented inclusive sum in a fixed order (rows of 16 by row_shr 1/2/4/8, then the
-      // row totals by row_bcast 15/31): each segment's last lane ends with its total
-      v = __builtin_fmaf(dpp_shr<0x111>(v), m1, v);
-      v = __builtin_fmaf(dpp_shr<0x112>(v), m2, v);
-      v = __builtin_fmaf(dpp_shr<0x114>(v), m4, v);
-      v = __builtin_fmaf(dpp_shr<0x118>(v), m8, v);
-      v = __builtin_fmaf(dpp_take<0x142, 0xa>(v), m15, v);
-      v = __builtin_fmaf(dpp_take<0x143, 0xc>(v), m31, v);
-      const int ia = j - base;
-      if (tail && v != 0.0f && (unsigned)ia < (unsigned)span) atomicAdd(&acc[ia], (double)v);   // one lane per address
+      v = seg_scan(v, m1, m2, m4, m8, m15, m31);
+      // a nonzero total has a lane of the segment on at k, so entry cj + k - base is in the row
+      if (tail && v != 0.0f) atomicAdd(acc0 + kStride * k, (double)v);   // one lane per address
     }
   }
   __syncthreads();
   double* row = P.core_part + ((size_t)blockIdx.y * P.ncwg + w) * span;
-  for (int i = lane; i < span; i += 64) row[i] = acc[i];
+  for (int i = lane; i < span; i += 64) row[i] = s_core[kStride * i];
 }
 
 // out[j] += the core rows covering bin j, in row order
