@@ -809,6 +809,7 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
     }
     const bool has = act && core.y > core.x;
     if (!__ballot(has)) continue;
+    const W4Region2 r2(y);
     // offsets k (bin cj + k) of this lane's core inside the row: [lo, hi)
     const int lo = has ? max(core.x, base) - cj : 0x3fffffff;
     const int hi = has ? min(core.y, base + span) - cj : -0x3fffffff;
@@ -834,7 +835,7 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
       float v = 0.0f;
       if (k >= lo && k < hi) {
         const double x = fabs(nu - fq0[kStride * k]) * g;
-        v = amp * w4_core_mixed(x, y);
+        v = amp * w4_core_mixed(x, y, r2);
       }
       v = seg_scan(v, m1, m2, m4, m8, m15, m31);
       // a nonzero total has a lane of the segment on at k, so entry cj + k - base is in the row

@@ -100,22 +100,47 @@ __device__ __forceinline__ double w4(double x, double y) {
 // FP32 forms used by the mixed-precision kernels.
 // ---------------------------------------------------------------------------
 
-// W4 regions II-IV for the mixed-precision core bins (|x| + y < 15), x >= 0.
-// Regions II and III are rational in t with no cancellation and run in FP32
-// (max 1.6e-6 relative against FP64 over their domains); region IV keeps the
-// FP64 t P(u)/Q(u) term, whose real part cancels in FP32 near |x| ~ 5, and
-// takes exp(u) = e^(y^2-x^2)(cos 2xy - i sin 2xy) from the FP32 hardware
-// exp/cos (2.7e-6 max).  The region is decided in FP64 by the caller's x, y.
-__device__ __forceinline__ float w4_core_mixed(double xd, double yd) {
-  const float x = (float)xd, y = (float)yd;
-  if (w4_sum(xd, yd) >= 5.5) {  // region II: t (1.410474 + 0.5641896 u) / (0.75 + u (3 + u))
-    const float ur = y * y - x * x, ui = -2.0f * x * y;
-    const float ar = 1.410474f + 0.5641896f * ur, ai = 0.5641896f * ui;
-    const float nr = y * ar + x * ai, ni = y * ai - x * ar;            // t * a, t = y - i x
-    const float br = 3.0f + ur, bi = ui;
-    const float dr = 0.75f + (ur * br - ui * bi), di = ur * bi + ui * br;
-    return (nr * dr + ni * di) * __builtin_amdgcn_rcpf(dr * dr + di * di);
+// Region II of W4, Re of t (a + b u) / (0.75 + u (3 + u)) with t = y - i x, u = t^2,
+// a = 1.410474, b = 0.5641896, as a rational in s = x^2 (the real part is even in x).
+// With Y = y^2, N = t (a + b u) and D = 0.75 + u (3 + u):
+//   Re D = s^2 + (-3 - 6Y) s + (0.75 + 3Y + Y^2),   Im D / x = 4y s - 2y (3 + 2Y),
+//   Re N = y (a + bY) - 3by s,                      Im N / x = b s - (a + 3bY),
+//   Re(N / D) = (Re N Re D + s (Im N / x)(Im D / x)) / (Re D^2 + s (Im D / x)^2).
+// The coefficients depend on the line only: formed in FP64 once per line and rounded
+// to FP32; per bin 14 FP32 operations (max 1.9e-6 relative against FP64 over region II,
+// the complex form in FP32 1.7e-6).
+struct W4Region2 {
+  float c0, c1, d0, d1, n0, n1, m0;
+  __device__ __forceinline__ explicit W4Region2(double y) {
+    const double a = 1.410474, b = 0.5641896, Y = y * y;
+    c0 = (float)(0.75 + 3.0 * Y + Y * Y);
+    c1 = (float)(-3.0 - 6.0 * Y);
+    d0 = (float)(-2.0 * y * (3.0 + 2.0 * Y));
+    d1 = (float)(4.0 * y);
+    n0 = (float)(y * (a + b * Y));
+    n1 = (float)(-3.0 * b * y);
+    m0 = (float)(-(a + 3.0 * b * Y));
   }
+  __device__ __forceinline__ float operator()(float x) const {
+    const float s = x * x;
+    const float dr = (s + c1) * s + c0;
+    const float e = d1 * s + d0;                    // Im D / x
+    const float nr = n1 * s + n0;
+    const float ni = 0.5641896f * s + m0;           // Im N / x
+    const float se = s * e;
+    return (nr * dr + se * ni) * __builtin_amdgcn_rcpf(dr * dr + se * e);
+  }
+};
+
+// W4 regions II-IV for the mixed-precision core bins (|x| + y < 15), x >= 0, of the
+// line whose region-II coefficients are r2.  Regions II and III are rational in t with
+// no cancellation and run in FP32 (max 1.9e-6 relative against FP64 over their domains);
+// region IV keeps the FP64 t P(u)/Q(u) term, whose real part cancels in FP32 near
+// |x| ~ 5, and takes exp(u) = e^(y^2-x^2)(cos 2xy - i sin 2xy) from the FP32 hardware
+// exp/cos (2.7e-6 max).  The region is decided in FP64 by the caller's x, y.
+__device__ __forceinline__ float w4_core_mixed(double xd, double yd, const W4Region2& r2) {
+  const float x = (float)xd, y = (float)yd;
+  if (w4_sum(xd, yd) >= 5.5) return r2(x);        // region II
   if (w4_region3(xd, yd)) {  // region III
     const float tr = y, ti = -x;
     float nr = tr * 0.5642236f, ni = ti * 0.5642236f;
