@@ -273,9 +273,10 @@ void MixedRunner::release() {
   d_tasks = nullptr; partial = nullptr; counter = nullptr; starts = nullptr; core_part = nullptr;
   task_cap = 0; partial_cap = 0; cus = 0; starts_cap = 0; core_cap = 0;
   hipFree(far_geom); hipFree(far_rec); hipFree(far_rng); hipFree(edge_rng); hipFree(far_list); hipFree(far_cnt);
-  hipFree(far_evals); hipFree(far_kind);
+  hipFree(far_evals); hipFree(far_kind); hipFree(far_first);
   far_geom = nullptr; far_rec = nullptr; far_rng = nullptr; edge_rng = nullptr; far_list = nullptr; far_cnt = nullptr;
   far_evals = nullptr; far_kind = nullptr; far_geom_cap = far_rec_cap = far_blk_cap = 0; far_kind_cap = 0;
+  far_first = nullptr; far_first_cap = 0;
   for (auto& ev : far_ev) {
     if (ev) hipEventDestroy(ev);
     ev = nullptr;
@@ -434,7 +435,9 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     plan.far_nclus = tot;
     const size_t need_rec = (size_t)ntasks * tot * kFarRec;
     const size_t need_blk = (size_t)ntasks * nblocks;
-    if ((size_t)tot > far_geom_cap || need_rec > far_rec_cap || need_blk > far_blk_cap || ntasks > far_kind_cap) {
+    const int need_first = gv.points + 1;
+    if ((size_t)tot > far_geom_cap || need_rec > far_rec_cap || need_blk > far_blk_cap || ntasks > far_kind_cap ||
+        need_first > far_first_cap) {
       if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
       if ((size_t)tot > far_geom_cap) {
         hipFree(far_geom);
@@ -473,6 +476,14 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
         if ((e = hipMalloc(&far_kind, sizeof(int) * (size_t)ntasks))) return hip_fail(e, "hipMalloc(far kinds)");
         far_kind_cap = ntasks;
       }
+      if (need_first > far_first_cap) {
+        hipFree(far_first);
+        far_first = nullptr;
+        far_first_cap = 0;
+        if ((e = hipMalloc(&far_first, 2 * sizeof(int) * (size_t)need_first)))
+          return hip_fail(e, "hipMalloc(far bin bounds)");
+        far_first_cap = need_first;
+      }
     }
     plan.far = 1;
     plan.far_bins = bins_per_block();
@@ -483,6 +494,9 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     plan.edge_rng = edge_rng;
     plan.far_list = far_list;
     plan.far_cnt = far_cnt;
+    plan.far_ie1 = far_first;
+    plan.far_ib1 = far_first + far_first_cap;
+    plan.far_bin_cap = far_first_cap;
     plan.far_evals = far_evals;
   }
   return GX_OK;
@@ -492,7 +506,8 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
 size_t far_field_bytes(int ntasks, long long nlines, int nblocks) {
   const long long clusters = 2 * ((nlines + kFarLeaf - 1) / kFarLeaf) + 64;
   return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * kFarRec * sizeof(double)) +
-         (size_t)ntasks * nblocks * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + sizeof(int) * (size_t)ntasks;
+         (size_t)ntasks * nblocks * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + sizeof(int) * (size_t)ntasks +
+         2 * sizeof(int) * ((size_t)nblocks * 64 + 1);    // per-bin line bounds (callers count 64-bin blocks)
 }
 
 int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges, const int* ib, const int* ie,

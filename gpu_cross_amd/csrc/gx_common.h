@@ -98,6 +98,8 @@ struct MixedRunner {
   size_t far_blk_cap = 0;      // (task, block) slots
   int* far_kind = nullptr;
   int far_kind_cap = 0;        // tasks
+  int* far_first = nullptr;    // first line above each bin: ie (entries [0, cap)), then ib
+  int far_first_cap = 0;       // entries per array
   unsigned long long* far_evals = nullptr;
   hipEvent_t far_ev[2] = {nullptr, nullptr};
   bool far_timed = false;      // the last launch recorded far_ev

@@ -342,14 +342,32 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
 
 __global__ void k_far_reset(unsigned long long* c) { *c = 0ull; }
 
-// First index in [a, z) with v[index] >= key (v non-decreasing there).
-__device__ __forceinline__ int lower_bound_int(const int* __restrict__ v, int a, int z, int key) {
+// First index in [a, z) with v[index] > key (v non-decreasing there).
+__device__ __forceinline__ int upper_bound_int(const int* __restrict__ v, int a, int z, int key) {
   while (a < z) {
     const int m = a + (z - a) / 2;
-    if (v[m] < key) a = m + 1;
+    if (v[m] <= key) a = m + 1;
     else z = m;
   }
   return a;
+}
+
+// First line of [a, z) with ie > j (IB: ib > j), j in [gv.start_idx - 1, gv.start_idx + gv.points - 1]:
+// the per-bin first line over the whole sorted list (k_bin_first), clamped to [a, z) (exact,
+// ib and ie being non-decreasing over the list).
+template <bool IB>
+__device__ __forceinline__ int first_above(const MixedPlan& P, int a, int z, int j) {
+  const int* t = IB ? P.far_ib1 : P.far_ie1;
+  return min(max(t[j - P.gv.start_idx + 1], a), z);
+}
+
+// The per-bin first lines: one thread per bin j = gv.start_idx - 1 + t, t in [0, gv.points].
+__global__ __launch_bounds__(256) void k_bin_first(MixedPlan P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > P.gv.points) return;
+  const int j = P.gv.start_idx - 1 + t;
+  P.far_ie1[t] = upper_bound_int(P.ie, 0, P.nlines, j);
+  P.far_ib1[t] = upper_bound_int(P.ib, 0, P.nlines, j);
 }
 
 // Whether a series of P orders about centre nu_c (+ i gamma_c), radius r, converges within
@@ -429,8 +447,8 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
   if (w1 <= gv.start_idx + gv.points && rg.y > rg.x) {
     const double f_lo = gv.freq[w0], f_hi = gv.freq[w1 - 1];
     // lines whose windows cover the whole block: ie >= w1 and ib <= w0 (ib, ie rise with nu)
-    const int L1 = lower_bound_int(P.ie, rg.x, rg.y, w1);
-    const int L2 = lower_bound_int(P.ib, rg.x, rg.y, w0 + 1);
+    const int L1 = first_above<false>(P, rg.x, rg.y, w1 - 1);
+    const int L2 = first_above<true>(P, rg.x, rg.y, w0);
     const int leafL = (L1 + kFarLeaf - 1) / kFarLeaf, leafR = L2 / kFarLeaf;
     const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
     int* list = P.far_list + slot * kFarList;
@@ -502,10 +520,10 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
   const int2 rg = P.ranges[b];
   if (w1 <= gv.start_idx + gv.points && rg.y > rg.x) {
     const double f_lo = gv.freq[w0], f_hi = gv.freq[w1 - 1];
-    const int L1 = lower_bound_int(P.ie, rg.x, rg.y, w1);        // ie >= w1 from here
-    const int L2 = lower_bound_int(P.ib, rg.x, rg.y, w0 + 1);    // ib > w0 from here
-    const int Lx = lower_bound_int(P.ie, rg.x, rg.y, w0 + 1);    // ie > w0 from here
-    const int Ly = lower_bound_int(P.ib, rg.x, rg.y, w1);        // ib >= w1 from here
+    const int L1 = first_above<false>(P, rg.x, rg.y, w1 - 1);    // ie >= w1 from here
+    const int L2 = first_above<true>(P, rg.x, rg.y, w0);         // ib > w0 from here
+    const int Lx = first_above<false>(P, rg.x, rg.y, w0);        // ie > w0 from here
+    const int Ly = first_above<true>(P, rg.x, rg.y, w1 - 1);     // ib >= w1 from here
     const int em0 = Lx, em1 = min(L1, L2) / kFarLeaf * kFarLeaf;
     const int ep0 = (max(L1, L2) + kFarLeaf - 1) / kFarLeaf * kFarLeaf, ep1 = Ly;
     const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
@@ -566,16 +584,6 @@ __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
   }
 #pragma unroll
   for (int q = 0; q < R; ++q) tk.out[w0 + lane + 64 * q] += acc[q];
-}
-
-// First index in [a, z) with v[index] > key (v non-decreasing there).
-__device__ __forceinline__ int upper_bound_int(const int* __restrict__ v, int a, int z, int key) {
-  while (a < z) {
-    const int m = a + (z - a) / 2;
-    if (v[m] <= key) a = m + 1;
-    else z = m;
-  }
-  return a;
 }
 
 // Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
@@ -855,14 +863,14 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   unsigned long long pairs = 0;
   double acc = 0.0;
   if (er.x < er.y) {                                           // er.y is a leaf boundary
-    const int s0 = upper_bound_int(P.ie, er.x, er.y, j);       // ie > j from here
+    const int s0 = first_above<false>(P, er.x, er.y, j);       // ie > j from here
     const int A = (s0 + kFarLeaf - 1) / kFarLeaf;
     pairs += (unsigned long long)(er.y - s0);
     acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
     acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red);
   }
   if (er.z < er.w) {                                           // er.z is a leaf boundary
-    const int p1 = upper_bound_int(P.ib, er.z, er.w, j);       // ib <= j before here
+    const int p1 = first_above<true>(P, er.z, er.w, j);        // ib <= j before here
     const int D = p1 / kFarLeaf;
     pairs += (unsigned long long)(p1 - er.z);
     acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
@@ -879,12 +887,14 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
 
 hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
+  if (plan.far_bin_cap < plan.gv.points + 1) return hipErrorInvalidValue;   // sized for another grid
   k_far_geom<<<(plan.far_nclus + 255) / 256, 256, 0, s>>>(plan, plan.far_geom);
   k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
   k_far_kind<<<plan.ntasks, 256, 0, s>>>(plan);
   k_far_leaf<<<dim3((plan.far_n[0] + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
+  k_bin_first<<<(plan.gv.points + 1 + 255) / 256, 256, 0, s>>>(plan);
   k_far_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
   k_edge_plan<<<dim3((plan.nblocks + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
   return hipGetLastError();

@@ -138,6 +138,9 @@ struct MixedPlan {
   int4* far_rng;           // [ntasks][nblocks]: far line ranges [x, y) and [z, w) of each block
   int4* edge_rng;          // [ntasks][nblocks]: window-end lines from moments: [x, y) end inside the
                            // block (bins j < ie), [z, w) start inside it (bins j >= ib)
+  int* far_ie1;            // [gv.points + 1]: first line with ie > gv.start_idx - 1 + t (ie, ib are
+  int* far_ib1;            // non-decreasing over the sorted lines); far_ib1 likewise for ib
+  int far_bin_cap;         // entries of far_ie1 / far_ib1
   int* far_list;           // [ntasks][nblocks][kFarList]: clusters of each block, level << 24 | index
   int* far_cnt;            // [ntasks][nblocks]
   unsigned long long* far_evals;   // line-bin pairs taken from moments
