@@ -320,3 +320,39 @@ def test_voigt_mixed_bins_per_lane_and_repeatability(voigt_lines, monkeypatch, R
     print(f"  max rel err {err:.3e}, core evals {t.core_evals} of {t.evals}", flush=True)
     assert err <= MIXED_TOL
     assert 0 < t.core_evals < t.evals
+
+
+@pytest.mark.parametrize("T,P,x", [(296.0, 1.0, 0.79), (1200.0, 0.05, 1.0)])
+def test_hitran_par_through_manager(T, P, x):
+    """HITRAN .par text read by the library's reader (gamma = (gamma_air x + gamma_self
+    (1 - x)) P, A from the intensity for A = 0 records, the driver's range filter) and then
+    through the FP64 manager, against the oracle's cross-section of the lines read; the
+    mixed path against the same at its own tolerance.  VoigtProfile.cpp:56-107 is the
+    reference's loop for this input."""
+    from gpu_cross_amd import ingest
+    rng = np.random.default_rng(11)
+    n = 30000
+    nu = np.sort(rng.uniform(1950.0, 2250.0, n))
+    A = 10 ** rng.uniform(-3, 2, n)
+    A[::7] = 0.0
+    text = bytes(ingest.format_hitran(nu, 10 ** rng.uniform(-24, -19, n), A, rng.uniform(0.02, 0.1, n),
+                                      rng.uniform(0.05, 0.4, n), rng.uniform(0, 5000, n), rng.uniform(0.4, 0.8, n),
+                                      rng.integers(1, 60, n).astype(float), rng.integers(1, 60, n).astype(float),
+                                      mol_id=5, iso=1))
+    start, end, cutoff = 2000.0, 2200.0, 25.0
+    L, mol, iso = ingest.read_hitran(text, pressure=P, mixture=x, ref_partition=107.4, lo=start - cutoff,
+                                     hi=end + cutoff)
+    assert (mol, iso) == (5, 1) and L.n > 20000 and np.all(L.aif > 0)
+    freq, d = make_grid(start, end, 20000)
+    Q = 107.4 * (T / 296.0)                       # Q(T) from the caller (TIPS is not part of the library)
+    print(f"hitran lines={L.n} T={T} P={P} x={x}", flush=True)
+    out, t = run_manager(capi.VOIGT, L, freq, d, T, Q, 28.0, P, gamma=L.gamma, ref_temp=296.0, cutoff=cutoff)
+    c = po.consts(po.VOIGT, T, Q, d, 28.0, P, 296.0, cutoff=cutoff)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, L.gamma, L.nexp, freq, freq.size, 0, 4)
+    err = rel_err(out, ref)
+    mixed, _ = run_manager(capi.VOIGT, L, freq, d, T, Q, 28.0, P, gamma=L.gamma, ref_temp=296.0, cutoff=cutoff,
+                           precision=capi.PREC_MIXED)
+    err_m = rel_err(mixed, ref)
+    print(f"  fp64 max rel err {err:.3e}, mixed {err_m:.3e}", flush=True)
+    assert err <= FP64_TOL and err_m <= MIXED_TOL
+    assert t.evals == po.count_evals(c, L.nu, freq, freq.size, 0)
