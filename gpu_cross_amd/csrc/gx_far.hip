@@ -598,19 +598,20 @@ __device__ __forceinline__ double region1_line(double nu, double g, double y, fl
 __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedTask& tk, int a, int z, double f) {
   double s = 0.0;
   int l = a;
-  // four lines' loads issued together, then their terms added in line order
-  for (; l + 4 <= z; l += 4) {
-    double nu[4], g[4], y[4];
-    float amp[4];
+  // eight lines' loads issued together (the loop is bound by their latency: four per batch
+  // made the window-end kernels ~9 ms slower per table), then their terms added in line order
+  for (; l + 8 <= z; l += 8) {
+    double nu[8], g[8], y[8];
+    float amp[8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 8; ++i) {
       nu[i] = P.nu[l + i];
       g[i] = tk.g[l + i];
       y[i] = tk.y[l + i];
       amp[i] = tk.amp[l + i];
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s += region1_line(nu[i], g[i], y[i], amp[i], f);
+    for (int i = 0; i < 8; ++i) s += region1_line(nu[i], g[i], y[i], amp[i], f);
   }
   for (; l < z; ++l) s += region1_line(P.nu[l], tk.g[l], tk.y[l], tk.amp[l], f);
   return s;
