@@ -812,14 +812,16 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
   const int own1 = SUFFIX ? (tid + 1 < NB ? s_cut[tid + 1] : hi) : hi;
   if (U0 >= U1) return 0.0;
   const double cE = 0.5 * (P.far_geom[U0].y + P.far_geom[U1 - 1].z);
-  double glo = 1e300, ghi = -1e300;
-  for (int L = U0 + tid; L < U1; L += NB) {
-    const double* rc = recs + (size_t)L * kFarRec;
-    glo = fmin(glo, rc[3]);
-    ghi = fmax(ghi, rc[4]);
+  double glo = 1e300, ghi = -1e300;                            // Lorentz widths: complex centres only
+  if (CPLX) {
+    for (int L = U0 + tid; L < U1; L += NB) {
+      const double* rc = recs + (size_t)L * kFarRec;
+      glo = fmin(glo, rc[3]);
+      ghi = fmax(ghi, rc[4]);
+    }
+    glo = block_extreme<NW, false>(glo, s_red);
+    ghi = block_extreme<NW, true>(ghi, s_red);
   }
-  glo = block_extreme<NW, false>(glo, s_red);
-  ghi = block_extreme<NW, true>(ghi, s_red);
   const double gE = CPLX ? 0.5 * (glo + ghi) : 0.0;
   double rE = 0.0;
   for (int L = U0 + tid; L < U1; L += NB) {
