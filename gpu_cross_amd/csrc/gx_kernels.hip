@@ -196,6 +196,7 @@ __device__ __forceinline__ bool in_core(double nu, double g, double y, const dou
 // The temperature-dependent part of one line at one point: S, 1/alpha and (T_ref/T)^n.
 struct PrepT {
   double S, g, tscale;
+  double alpha;        // nu dpwcoef = 1/g up to rounding: only for the core bracket
 };
 
 __device__ __forceinline__ PrepT prep_mixed_t(const PointConsts& pc, double nu_if, double a, int gi, double elow,
@@ -205,7 +206,8 @@ __device__ __forceinline__ PrepT prep_mixed_t(const PointConsts& pc, double nu_i
   t.S = pc.cmcoef * a * gi * kIsqrtPi * exp(-pc.beta * elow) * (1.0 - exp(-pc.beta * nu_if)) /
         (nu_if * nu_if * pc.partition);
   if (pc.convention == 1 && gi == -1) t.S = a;
-  t.g = 1.0 / (nu_if * pc.dpwcoef);
+  t.alpha = nu_if * pc.dpwcoef;
+  t.g = 1.0 / t.alpha;
   t.tscale = pow(pc.ref_temp / pc.temperature, nexp);
   return t;
 }
@@ -219,16 +221,18 @@ __device__ __forceinline__ bool same_t(const PointConsts& a, const PointConsts& 
 // The pressure-dependent part and the outputs: y, amp, the core range.
 __device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts& pc, const PrepT& t, double nu_if,
                                                                double gamma, int ib, int ie, const GridView& gv,
-                                                               const MixedLineData& out, int i, int p) {
+                                                               double inv_dfreq, const MixedLineData& out, int i,
+                                                               int p) {
 #pragma clang fp contract(off)
   const double g = t.g;
   const double gammaL = (gamma * pc.pressure_scale) * t.tscale;
   const double y = gammaL * g;
   int k0 = 0, k1 = 0;
   if (y < 15.0 && ie > ib) {
-    const double half = (15.0 - y) / g;
-    const double lo = floor((nu_if - half - gv.start_nu) / gv.dfreq) - 1.0;
-    const double hi = ceil((nu_if + half - gv.start_nu) / gv.dfreq) + 1.0;
+    // bracket from products (rounding far below the one-bin margin), trimmed by the exact test
+    const double half = (15.0 - y) * t.alpha;
+    const double lo = floor((nu_if - half - gv.start_nu) * inv_dfreq) - 1.0;
+    const double hi = ceil((nu_if + half - gv.start_nu) * inv_dfreq) + 1.0;
     int a0 = (int)fmax(lo, (double)ib), z = (int)fmin(hi, (double)ie);
     while (a0 < z && !in_core(nu_if, g, y, gv.freq, a0)) ++a0;
     while (z > a0 && !in_core(nu_if, g, y, gv.freq, z - 1)) --z;
@@ -259,10 +263,11 @@ __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restric
     const double nu_if = nu == 0 ? 1e-6 : nu;
     const double a = lv.aif[i], elow = lv.elow[i], gamma = lv.gamma[i], nexp = lv.nexp[i];
     const int gi = lv.gns[i], ib = IB[i], ie = IE[i];
+    const double inv_dfreq = 1.0 / gv.dfreq;
     PrepT t = prep_mixed_t(pcs[0], nu_if, a, gi, elow, nexp);
     for (int p = 0; p < npts; ++p) {
       if (p > 0 && !same_t(pcs[p], pcs[p - 1])) t = prep_mixed_t(pcs[p], nu_if, a, gi, elow, nexp);
-      w += prep_mixed_point(pcs[p], t, nu_if, gamma, ib, ie, gv, out, i, p);
+      w += prep_mixed_point(pcs[p], t, nu_if, gamma, ib, ie, gv, inv_dfreq, out, i, p);
     }
   }
   if (core_evals) block_count(w, core_evals);
