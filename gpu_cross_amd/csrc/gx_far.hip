@@ -82,24 +82,29 @@ __global__ __launch_bounds__(256) void k_far_kind(MixedPlan P) {
     P.far_kind[blockIdx.x] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3])) >= kFarCplxGamma ? 1 : 0;
 }
 
-// 64-bit DPP move (every source lane valid for the controls used here).
+// FP32 bound of a non-negative FP64 value, rounded up (UP) or down: a maximum or minimum of
+// such bounds over the wave bounds the FP64 one (one ulp step on the bits, v >= 0).
+template <bool UP>
+__device__ __forceinline__ float f32_bound(double v) {
+  float f = (float)v;
+  if (UP ? (double)f < v : (double)f > v) f = __int_as_float(__float_as_int(f) + (UP ? 1 : -1));
+  return f;
+}
+
 template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+__device__ __forceinline__ float dpp_f32(float v) {   // every source lane valid for these controls
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
 // Maximum (MAX) or minimum over the wave, in every lane: quad swaps and rotations by 4 and
 // 8 inside each row of 16 lanes, then across rows.
 template <bool MAX>
-__device__ __forceinline__ double wave_extreme(double v) {
-  auto pick = [](double a, double b) { return MAX ? fmax(a, b) : fmin(a, b); };
-  v = pick(v, dpp_f64<0xB1>(v));          // quad_perm [1, 0, 3, 2]
-  v = pick(v, dpp_f64<0x4E>(v));          // quad_perm [2, 3, 0, 1]
-  v = pick(v, dpp_f64<0x124>(v));         // row_ror 4
-  v = pick(v, dpp_f64<0x128>(v));         // row_ror 8
+__device__ __forceinline__ float wave_extreme_f32(float v) {
+  auto pick = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+  v = pick(v, dpp_f32<0xB1>(v));          // quad_perm [1, 0, 3, 2]
+  v = pick(v, dpp_f32<0x4E>(v));          // quad_perm [2, 3, 0, 1]
+  v = pick(v, dpp_f32<0x124>(v));         // row_ror 4
+  v = pick(v, dpp_f32<0x128>(v));         // row_ror 8
   v = pick(v, __shfl_xor(v, 16));
   return pick(v, __shfl_xor(v, 32));
 }
@@ -158,19 +163,22 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
     y = tk.y[l];
     amp = tk.amp[l];
   }
-  const double al = act ? 1.0 / g : 0.0;
+  const double al = act ? nu * tk.dpwcoef : 0.0;               // alpha: g = 1 / (nu dpwcoef)
   const double s = kPoleShift * al, im = y * al;               // im: the line's gamma
   const double dp = nu + s - c, dm = nu - s - c;
-  const double amax = wave_extreme<true>(al);
-  const double ymin = wave_extreme<false>(act ? y : 1e300);
+  // the bounds' extremes from FP32 values rounded outward (all are >= 0): the maxima can
+  // only grow and the minima shrink, so every test built on them stays conservative
+  const double amax = (double)wave_extreme_f32<true>(f32_bound<true>(al));
+  const double ymin = (double)wave_extreme_f32<false>(act ? f32_bound<false>(y) : INFINITY);
   double glo = 0.0, ghi = 0.0;                                 // gamma range: complex centres only
   if (cplx) {
-    glo = wave_extreme<false>(act ? im : 1e300);
-    ghi = wave_extreme<true>(act ? im : -1e300);
+    glo = (double)wave_extreme_f32<false>(act ? f32_bound<false>(im) : INFINITY);
+    ghi = (double)wave_extreme_f32<true>(act ? f32_bound<true>(im) : 0.0f);
   }
   const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
   const double wim = im - gc;                                  // Im (zeta - centre)
-  const double r2 = wave_extreme<true>(act ? fmax(dp * dp, dm * dm) + wim * wim : 0.0);
+  const double r2 =
+      (double)wave_extreme_f32<true>(act ? f32_bound<true>(fmax(dp * dp, dm * dm) + wim * wim) : 0.0f);
   double r = sqrt(r2);
   if (!(r > 0.0)) r = 1.0;
   const double ir = 1.0 / r;
