@@ -139,8 +139,11 @@ __device__ __forceinline__ void wave_sums(T (&v)[N], int lane) {
     for (int i = 0; i < N / 8; ++i) v[i] += __shfl_xor(v[i], d);
 }
 
-// Leaf moments of every task: one wave per (leaf, task), one line per lane, the lanes'
-// terms summed by wave_sums.  The radius, centre and the extremes the truncation bounds
+// Points per wave of the leaf kernel: each wave reads its lines' nu once for them.
+constexpr int kLeafTasks = 8;
+
+// Leaf moments of every task: one wave per (leaf, kLeafTasks points), one line per lane, the
+// lanes' terms summed by wave_sums.  The radius, centre and the extremes the truncation bounds
 // use are FP64; each line's scaled pole offsets w (|w| <= 1) and its weight go to FP32, and
 // the powers and the wave sums run in FP32: a moment's rounding error is then ~1e-7 of the
 // sum of its terms' magnitudes, and the series weighs moment k by at most eta^(k+1), so the
@@ -150,92 +153,94 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   const int lane = threadIdx.x & 63;
   const int leaf = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (leaf >= P.far_n[0]) return;
-  const MixedTask tk = P.tasks[blockIdx.y];
-  const bool cplx = P.far_kind[blockIdx.y] != 0;
   const int l = leaf * kFarLeaf + lane;
   const bool act = l < P.nlines;
   const double c = P.far_geom[leaf].x;
-  double nu = c, g = 1.0, y = 0.0;
-  float amp = 0.0f;
-  if (act) {
-    nu = nu_if(P.nu[l]);
-    g = tk.g[l];
-    y = tk.y[l];
-    amp = tk.amp[l];
-  }
-  const double al = act ? nu * tk.dpwcoef : 0.0;               // alpha: g = 1 / (nu dpwcoef)
-  const double s = kPoleShift * al, im = y * al;               // im: the line's gamma
-  const double dp = nu + s - c, dm = nu - s - c;
-  // the bounds' extremes from FP32 values rounded outward (all are >= 0): the maxima can
-  // only grow and the minima shrink, so every test built on them stays conservative
-  const double amax = (double)wave_extreme_f32<true>(f32_bound<true>(al));
-  const double ymin = (double)wave_extreme_f32<false>(act ? f32_bound<false>(y) : INFINITY);
-  double glo = 0.0, ghi = 0.0;                                 // gamma range: complex centres only
-  if (cplx) {
-    glo = (double)wave_extreme_f32<false>(act ? f32_bound<false>(im) : INFINITY);
-    ghi = (double)wave_extreme_f32<true>(act ? f32_bound<true>(im) : 0.0f);
-  }
-  const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
-  const double wim = im - gc;                                  // Im (zeta - centre)
-  const double r2 =
-      (double)wave_extreme_f32<true>(act ? f32_bound<true>(fmax(dp * dp, dm * dm) + wim * wim) : 0.0f);
-  double r = sqrt(r2);
-  if (!(r > 0.0)) r = 1.0;
-  const double ir = 1.0 / r;
-  const float kap = act ? (float)(kKappa * (double)amp * al) : 0.0f;   // amp = S g
-  const float wi = (float)(wim * ir);
-  double* rec = P.far_rec + ((size_t)blockIdx.y * P.far_nclus + leaf) * kFarRec;
-  if (lane == 0) {
-    rec[0] = r;
-    rec[1] = amax;
-    rec[2] = ymin;
-    rec[3] = glo;
-    rec[4] = ghi;
-    rec[5] = gc;
-  }
-  // every lane's terms for all moments, then one halving reduction over the wave; lanes
-  // 8m (m < 8) write the totals they hold
-  const float wrp = (float)(dp * ir), wrm = (float)(dm * ir);
-  const int first = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-  if (!cplx) {
-    // Im w^k of each pole by the real recurrence z_k = 2 Re(w) z_(k-1) - |w|^2 z_(k-2)
-    // (w and its conjugate are the roots of t^2 - 2 Re(w) t + |w|^2), z_0 = 0, z_1 = Im w
-    float v[kFarP];
-    const float tp = 2.0f * wrp, np = wrp * wrp + wi * wi;
-    const float tm = 2.0f * wrm, nm = wrm * wrm + wi * wi;
-    float p1 = wi, p0 = 0.0f, m1 = wi, m0 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kFarP; ++k) {
-      v[k] = kap * (p1 + m1);
-      const float p2 = tp * p1 - np * p0, m2 = tm * m1 - nm * m0;
-      p0 = p1;
-      p1 = p2;
-      m0 = m1;
-      m1 = m2;
+  const double nu = act ? nu_if(P.nu[l]) : c;                  // read once for kLeafTasks points
+  const int t1 = min(P.ntasks, (int)(blockIdx.y + 1) * kLeafTasks);
+  for (int task = blockIdx.y * kLeafTasks; task < t1; ++task) {
+    const MixedTask tk = P.tasks[task];
+    const bool cplx = P.far_kind[task] != 0;
+    double y = 0.0;
+    float amp = 0.0f;
+    if (act) {
+      y = tk.y[l];
+      amp = tk.amp[l];
     }
-    wave_sums(v, lane);
-    if ((lane & 7) == 0) {
-#pragma unroll
-      for (int t = 0; t < kFarP / 8; ++t) rec[kFarHead + first * (kFarP / 8) + t] = (double)v[t];
+    const double al = act ? nu * tk.dpwcoef : 0.0;               // alpha: g = 1 / (nu dpwcoef)
+    const double s = kPoleShift * al, im = y * al;               // im: the line's gamma
+    const double dp = nu + s - c, dm = nu - s - c;
+    // the bounds' extremes from FP32 values rounded outward (all are >= 0): the maxima can
+    // only grow and the minima shrink, so every test built on them stays conservative
+    const double amax = (double)wave_extreme_f32<true>(f32_bound<true>(al));
+    const double ymin = (double)wave_extreme_f32<false>(act ? f32_bound<false>(y) : INFINITY);
+    double glo = 0.0, ghi = 0.0;                                 // gamma range: complex centres only
+    if (cplx) {
+      glo = (double)wave_extreme_f32<false>(act ? f32_bound<false>(im) : INFINITY);
+      ghi = (double)wave_extreme_f32<true>(act ? f32_bound<true>(im) : 0.0f);
     }
-  } else {
-    float v[2 * kFarP];                                        // (Re, Im) of M_0..M_(P-1)
-    float pr = 1.0f, pi = 0.0f, mr = 1.0f, mi = 0.0f;          // w+^0, w-^0
-#pragma unroll
-    for (int k = 0; k < kFarP; ++k) {
-      v[2 * k] = kap * (pr + mr);
-      v[2 * k + 1] = kap * (pi + mi);
-      const float nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
-      const float n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
-      pr = nr;
-      pi = ni;
-      mr = n2r;
-      mi = n2i;
+    const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
+    const double wim = im - gc;                                  // Im (zeta - centre)
+    const double r2 =
+        (double)wave_extreme_f32<true>(act ? f32_bound<true>(fmax(dp * dp, dm * dm) + wim * wim) : 0.0f);
+    double r = sqrt(r2);
+    if (!(r > 0.0)) r = 1.0;
+    const double ir = 1.0 / r;
+    const float kap = act ? (float)(kKappa * (double)amp * al) : 0.0f;   // amp = S g
+    const float wi = (float)(wim * ir);
+    double* rec = P.far_rec + ((size_t)task * P.far_nclus + leaf) * kFarRec;
+    if (lane == 0) {
+      rec[0] = r;
+      rec[1] = amax;
+      rec[2] = ymin;
+      rec[3] = glo;
+      rec[4] = ghi;
+      rec[5] = gc;
     }
-    wave_sums(v, lane);
-    if ((lane & 7) == 0) {
+    // every lane's terms for all moments, then one halving reduction over the wave; lanes
+    // 8m (m < 8) write the totals they hold
+    const float wrp = (float)(dp * ir), wrm = (float)(dm * ir);
+    const int first = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    if (!cplx) {
+      // Im w^k of each pole by the real recurrence z_k = 2 Re(w) z_(k-1) - |w|^2 z_(k-2)
+      // (w and its conjugate are the roots of t^2 - 2 Re(w) t + |w|^2), z_0 = 0, z_1 = Im w
+      float v[kFarP];
+      const float tp = 2.0f * wrp, np = wrp * wrp + wi * wi;
+      const float tm = 2.0f * wrm, nm = wrm * wrm + wi * wi;
+      float p1 = wi, p0 = 0.0f, m1 = wi, m0 = 0.0f;
 #pragma unroll
-      for (int t = 0; t < kFarP / 4; ++t) rec[kFarHead + first * (kFarP / 4) + t] = (double)v[t];
+      for (int k = 0; k < kFarP; ++k) {
+        v[k] = kap * (p1 + m1);
+        const float p2 = tp * p1 - np * p0, m2 = tm * m1 - nm * m0;
+        p0 = p1;
+        p1 = p2;
+        m0 = m1;
+        m1 = m2;
+      }
+      wave_sums(v, lane);
+      if ((lane & 7) == 0) {
+#pragma unroll
+        for (int t = 0; t < kFarP / 8; ++t) rec[kFarHead + first * (kFarP / 8) + t] = (double)v[t];
+      }
+    } else {
+      float v[2 * kFarP];                                        // (Re, Im) of M_0..M_(P-1)
+      float pr = 1.0f, pi = 0.0f, mr = 1.0f, mi = 0.0f;          // w+^0, w-^0
+#pragma unroll
+      for (int k = 0; k < kFarP; ++k) {
+        v[2 * k] = kap * (pr + mr);
+        v[2 * k + 1] = kap * (pi + mi);
+        const float nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
+        const float n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
+        pr = nr;
+        pi = ni;
+        mr = n2r;
+        mi = n2i;
+      }
+      wave_sums(v, lane);
+      if ((lane & 7) == 0) {
+#pragma unroll
+        for (int t = 0; t < kFarP / 4; ++t) rec[kFarHead + first * (kFarP / 4) + t] = (double)v[t];
+      }
     }
   }
 }
@@ -902,7 +907,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   k_far_geom<<<(plan.far_nclus + 255) / 256, 256, 0, s>>>(plan, plan.far_geom);
   k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
   k_far_kind<<<plan.ntasks, 256, 0, s>>>(plan);
-  k_far_leaf<<<dim3((plan.far_n[0] + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
+  k_far_leaf<<<dim3((plan.far_n[0] + 3) / 4, (plan.ntasks + kLeafTasks - 1) / kLeafTasks), 256, 0, s>>>(plan);
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
   k_bin_first<<<(plan.gv.points + 1 + 255) / 256, 256, 0, s>>>(plan);
