@@ -96,6 +96,10 @@ class OpacityInfo(ctypes.Structure):
                 ("cutoff", ctypes.c_double), ("data_offset", ctypes.c_uint64)]
 
 
+# scratch-array limits of gx_execute_noalloc (include/gx.h)
+NOALLOC_MAX_LINES = 1 << 20
+NOALLOC_MAX_BINS = 1 << 24
+
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
 
 
@@ -136,6 +140,7 @@ def load():
     _sig(L, "gx_transfer_freq", ctypes.c_int, [P, _dp, _dp, ctypes.c_int])
     _sig(L, "gx_transfer_vectors", ctypes.c_int, [P, ctypes.c_size_t, _dp, _dp, _dp, _ip, _dp, _dp])
     _sig(L, "gx_execute", ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int])
+    _sig(L, "gx_execute_noalloc", ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int])
     _sig(L, "gx_ready", ctypes.c_int, [P])
     _sig(L, "gx_transfer_results", ctypes.c_int, [P, _dp, _dp, ctypes.c_int])
     _sig(L, "gx_cleanup", ctypes.c_int, [P])

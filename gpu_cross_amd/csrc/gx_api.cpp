@@ -22,6 +22,7 @@
 #include "../../include/gx.h"
 #include "gx_internal.h"
 #include "gx_common.h"
+#include "gx_noalloc.h"
 
 namespace gx {
 
@@ -605,6 +606,8 @@ struct gx_ctx {
   MixedRunner mix;
   unsigned long long* d_evals = nullptr;   // kEvalSlots counters
   int last_lines = 0;
+  int noalloc_status = GX_OK;              // set-up of the device's scratch array (gx_execute_noalloc)
+  std::string noalloc_error;
 };
 
 namespace {
@@ -646,6 +649,9 @@ int gx_create(int device, int profile, size_t mem_budget, gx_ctx** out) {
     gx_destroy(c);
     return hip_fail(e, "hipMalloc");
   }
+  // the scratch array of gx_execute_noalloc: its address on this device, looked up once;
+  // a failure is kept for that entry point and does not fail the context
+  if ((c->noalloc_status = noalloc_setup(device)) != GX_OK) c->noalloc_error = g_last_error;
   *out = c;
   return GX_OK;
 }
@@ -738,6 +744,26 @@ int gx_execute(gx_ctx* c, int points, int n_ener, int start_idx) {
   c->last_mixed = k.precision == GX_PREC_MIXED && c->profile == GX_VOIGT && core_pass_fits(gv, pc.dpwcoef);
   return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->mix, c->d_intens, c->d_evals,
                    c->d_core_evals, c->stream, c->ev);
+}
+
+int gx_execute_noalloc(gx_ctx* c, int points, int n_ener, int start_idx) {
+  if (!c) return fail(GX_INVALID_ARG, "execute_noalloc: NULL context");
+  if (!c->have_consts) return fail(GX_INVALID_ARG, "execute_noalloc: initialize_constants not called");
+  if (!c->d_freq) return fail(GX_INVALID_ARG, "execute_noalloc: initialize_vectors not called");
+  if (points < 0 || start_idx < 0 || (long long)start_idx + points > c->M)
+    return fail(GX_INVALID_ARG, "execute_noalloc: bin range outside the grid");
+  if (n_ener < 0 || (size_t)n_ener > c->lines.n)
+    return fail(GX_INVALID_ARG, "execute_noalloc: N_ener exceeds the uploaded chunk");
+  if (c->consts.precision != GX_PREC_FP64)
+    return fail(GX_INVALID_ARG, "execute_noalloc: runs the FP64 path only (precision must be GX_PREC_FP64)");
+  if (c->noalloc_status != GX_OK) return fail(c->noalloc_status, c->noalloc_error);
+  int rc = set_device(c);
+  if (rc) return rc;
+  const gx_consts& k = c->consts;
+  PointConsts pc = make_point_consts(c->profile, k, 1.0);
+  GridView gv{c->d_freq, c->M, start_idx, points, c->start_nu, k.dfreq};
+  c->timed = false;   // gx_last_timing covers gx_execute only
+  return execute_noalloc(c->device, pc, gv, c->lines, n_ener, c->d_intens, c->stream);
 }
 
 int gx_ready(gx_ctx* c) {

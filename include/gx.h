@@ -110,6 +110,26 @@ int gx_transfer_vectors(gx_ctx *ctx, size_t nener, const double *energies, const
  * of the persistent grid.  Asynchronous. */
 int gx_execute(gx_ctx *ctx, int points, int n_ener, int start_idx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gx_execute on the FP64 path with the per-line
+ * intensities, widths and windows, the per-block line slices and the
+ * evaluation counters in one fixed device array of the library
+ * (GX_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held until the process
+ * ends) instead of buffers the context allocates.  Same kernels, so the same
+ * grid bit for bit.  The lines and the grid are those the context already
+ * holds (gx_transfer_vectors, gx_initialize_vectors).  Refuses more than
+ * GX_NOALLOC_MAX_LINES lines, more than GX_NOALLOC_MAX_BINS bins and
+ * GX_PREC_MIXED constants (GX_INVALID_ARG).  Calls no hipMalloc or hipFree and
+ * issues no memset.  Asynchronous on the context's stream; gx_last_timing does
+ * not cover it.  Every context on one GPU shares that GPU's array, which keeps
+ * the last call's contents: calls that overlap on one GPU (contexts on other
+ * streams, other host threads) race. */
+#define GX_NOALLOC_MAX_LINES (1 << 20)
+#define GX_NOALLOC_MAX_BINS (1 << 24)
+#define GX_NOALLOC_SCRATCH_BYTES ((size_t)34079232) /* 32 B per line, 8 B per 256 bins, 64 counters */
+int gx_execute_noalloc(gx_ctx *ctx, int points, int n_ener, int start_idx);
+
 /* ReadyForWork() (GPUManager.cpp:185-188): 1 idle, 0 busy, <0 error. */
 int gx_ready(gx_ctx *ctx);
 
