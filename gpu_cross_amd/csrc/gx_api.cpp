@@ -286,6 +286,13 @@ void MixedRunner::release() {
   h_tasks.clear();
 }
 
+size_t MixedRunner::bytes() const {
+  return (size_t)task_cap * sizeof(MixedTask) + partial_cap * sizeof(float2) + starts_cap * sizeof(int) +
+         core_cap * sizeof(double) + far_geom_cap * sizeof(double4) + far_rec_cap * sizeof(double) +
+         far_blk_cap * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + (size_t)far_kind_cap * sizeof(int) +
+         2 * sizeof(int) * (size_t)far_first_cap;
+}
+
 int MixedRunner::far_time(float* ms, unsigned long long* evals) {
   *ms = 0.0f;
   *evals = 0;
@@ -316,6 +323,33 @@ int core_kmax(const GridView& gv, double dpw) {
 }
 
 bool core_pass_fits(const GridView& gv, double dpw) { return gv.points <= 0 || core_kmax(gv, dpw) >= 0; }
+
+// Centre bins per workgroup from the grid alone (a core half-width estimate at
+// dpw = 1e-5), so that a task's rows, hence its sums, do not depend on the other
+// tasks of the launch.  Row boundaries sit at multiples of cpw in absolute bins
+// for the same reason.
+CoreGeom core_geometry(const GridView& gv, int kmax) {
+  CoreGeom g{};
+  const double kg = std::min(1e6, 15.0 * 1e-5 * chunk_max_abs_nu(gv) / gv.dfreq);
+  int cpw = (int)std::min(256.0, std::max(64.0, std::ceil(2.0 * kg / 64.0) * 64.0));
+  if (core_lds_bytes(cpw + 2 * kmax + 1) == 0) cpw = 64;
+  const long long lo = (long long)gv.start_idx - kmax - 1;
+  const long long cbase = (lo >= 0 ? lo / cpw : -((-lo + cpw - 1) / cpw)) * cpw;
+  const long long cend = (long long)gv.start_idx + gv.points + kmax + 1;
+  g.cpw = cpw;
+  g.span = cpw + 2 * kmax + 1;
+  g.cbase = (int)cbase;
+  g.ncwg = (int)((cend - cbase + cpw - 1) / cpw);
+  g.ncent = g.ncwg * cpw;
+  return g;
+}
+
+size_t core_rows_bytes(const GridView& gv, double dpw) {
+  const int kmax = core_kmax(gv, dpw);
+  if (kmax < 0 || gv.points <= 0) return 0;
+  const CoreGeom g = core_geometry(gv, kmax);
+  return (size_t)g.ncwg * (size_t)g.span * sizeof(double);
+}
 
 int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const GridView& gv, int nlines,
                          hipStream_t s) {
@@ -373,30 +407,20 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
 
   plan.nlines = std::max(nlines, 0);
   if (gv.points > 0 && plan.nlines > 0) {
-    const double d = gv.dfreq;
     int kmax = 0;
     for (int k = 0; k < ntasks; ++k) {
       const int kk = core_kmax(gv, tasks[k].dpwcoef);
       if (kk < 0) return fail(GX_INVALID_ARG, "mixed: W4 core too wide for the core pass; use FP64 for this grid");
       kmax = std::max(kmax, kk);
     }
-    // Centre bins per workgroup from the grid alone (a core half-width estimate at
-    // dpw = 1e-5), so that a task's rows, hence its sums, do not depend on the other
-    // tasks of the launch.  Row boundaries sit at multiples of cpw in absolute bins
-    // for the same reason.
-    const double kg = std::min(1e6, 15.0 * 1e-5 * chunk_max_abs_nu(gv) / d);
-    int cpw = (int)std::min(256.0, std::max(64.0, std::ceil(2.0 * kg / 64.0) * 64.0));
-    if (core_lds_bytes(cpw + 2 * kmax + 1) == 0) cpw = 64;
-    const int span = cpw + 2 * kmax + 1;
-    const long long lo = (long long)gv.start_idx - kmax - 1;
-    const long long cbase = (lo >= 0 ? lo / cpw : -((-lo + cpw - 1) / cpw)) * cpw;
-    const long long cend = (long long)gv.start_idx + gv.points + kmax + 1;
+    const CoreGeom cg = core_geometry(gv, kmax);
     plan.kmax = kmax;
-    plan.cpw = cpw;
-    plan.span = span;
-    plan.cbase = (int)cbase;
-    plan.ncwg = (int)((cend - cbase + cpw - 1) / cpw);
-    plan.ncent = plan.ncwg * cpw;
+    plan.cpw = cg.cpw;
+    plan.span = cg.span;
+    plan.cbase = cg.cbase;
+    plan.ncwg = cg.ncwg;
+    plan.ncent = cg.ncent;
+    const int span = cg.span;
     const size_t need_starts = (size_t)plan.ncent + 1;
     const size_t need_rows = (size_t)ntasks * plan.ncwg * span;
     if (need_starts > starts_cap || need_rows > core_cap) {

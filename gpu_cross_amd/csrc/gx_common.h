@@ -110,6 +110,8 @@ struct MixedRunner {
   // took from moments (call after the stream is synchronised).
   int far_time(float* ms, unsigned long long* evals);
   int bins_per_block() const { return mixed_bins_per_block(R); }
+  // Device bytes the runner holds now (its buffers only grow until release()).
+  size_t bytes() const;
   // Upload the task table, size the partials and the core-pass buffers for the
   // chunk gv of the grid and nlines sorted lines, and fix the launch shape.
   // May synchronise the stream; never call it inside a graph capture.
@@ -132,6 +134,17 @@ size_t far_field_bytes(int ntasks, long long nlines, int nblocks);
 double chunk_max_abs_nu(const GridView& gv);
 int core_kmax(const GridView& gv, double dpw);
 bool core_pass_fits(const GridView& gv, double dpw);
+// Row geometry of the core pass for the chunk gv and a core bound kmax (core_kmax).
+struct CoreGeom {
+  int cpw;     // centre bins per workgroup
+  int span;    // bins of one workgroup's row: cpw + 2 kmax + 1
+  int cbase;   // first centre bin that can reach the chunk
+  int ncwg;    // workgroups (rows) per task
+  int ncent;   // centre bins covered: ncwg * cpw
+};
+CoreGeom core_geometry(const GridView& gv, int kmax);
+// Bytes of one task's core-pass rows (ncwg * span doubles) at Doppler width coefficient dpw.
+size_t core_rows_bytes(const GridView& gv, double dpw);
 
 // Launch line prep, block ranges and accumulation of one point (asynchronous).
 // ev, when not NULL, holds 7 events: 4 around prep / ranges / accumulation and,

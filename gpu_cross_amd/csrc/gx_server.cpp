@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -43,7 +44,7 @@ struct Req {
   uint64_t id = 0;
   gx_request r{};
   int state = GX_REQ_QUEUED;
-  bool cancel = false;
+  std::atomic<bool> cancel{false};   // set by gx_server_cancel, read by the scheduler without the lock
   Clock::time_point t_submit, t_done;
   std::vector<double> host;
   long long evals = 0;
@@ -73,8 +74,8 @@ struct Slot {
   int M = 0;
 
   size_t footprint() const {
-    return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 + der.mcap * (size_t)der.mpoints * 12 +
-           (size_t)rng.cap * sizeof(int2) + run.partial_cap * sizeof(float2);
+    return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 + der.mcap * (size_t)der.mpoints * 28 +
+           (size_t)rng.cap * sizeof(int2) + run.bytes();
   }
   void release_buffers() {
     (void)hipFree(freq); (void)hipFree(out); (void)hipFree(ib); (void)hipFree(ie);
@@ -151,6 +152,34 @@ size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& b
   return need;
 }
 
+// Constants of one request on its own grid.
+PointConsts request_consts(const gx_server* S, const gx_request& q) {
+  gx_consts c{};
+  c.temperature = q.temperature;
+  c.partition = q.partition;
+  c.dfreq = (q.end_nu - q.start_nu) / double(q.npoints);   // BaseProfile.cpp:10
+  c.mean_mass = S->o.mean_mass;
+  c.pressure = q.pressure;
+  c.ref_temp = S->o.ref_temp;
+  c.cutoff = q.cutoff;
+  c.doppler_widths = q.doppler_widths;
+  c.convention = S->o.convention;
+  c.precision = q.precision;
+  return make_point_consts(q.profile, c, q.pressure);
+}
+
+// Widest per-line window of a batch: with per-line Doppler windows (k alpha + dfreq) it
+// grows with temperature, so the hottest request decides the line slice and block ranges
+// (a request's extra lines have windows that miss its grid and add nothing).
+double batch_cutmax(const gx_server* S, const std::vector<Req*>& batch) {
+  double c = 0.0;
+  for (const Req* q : batch) {
+    const PointConsts pc = request_consts(S, q->r);
+    c = std::max(c, max_cutoff(pc, S->L->ls.max_nu, (q->r.end_nu - q->r.start_nu) / double(q->r.npoints)));
+  }
+  return c;
+}
+
 // Line slice [lo, hi) of the sorted list that can reach grid [start, start + M*d) with cutoff c.
 void line_slice(const gx_lines* L, double start, double end, double cmax, double d, size_t& lo, size_t& hi) {
   const double a = start - cmax - 2 * d, b = end + cmax + 2 * d;
@@ -168,23 +197,9 @@ int launch_batch(gx_server* S, Slot& sl) {
   int rc;
   // per-request constants
   sl.h_pcs.resize(B);
-  for (int k = 0; k < B; ++k) {
-    const gx_request& q = sl.reqs[k]->r;
-    gx_consts c{};
-    c.temperature = q.temperature;
-    c.partition = q.partition;
-    c.dfreq = d;
-    c.mean_mass = S->o.mean_mass;
-    c.pressure = q.pressure;
-    c.ref_temp = S->o.ref_temp;
-    c.cutoff = q.cutoff;
-    c.doppler_widths = q.doppler_widths;
-    c.convention = S->o.convention;
-    c.precision = q.precision;
-    sl.h_pcs[k] = make_point_consts(q.profile, c, q.pressure);
-  }
+  for (int k = 0; k < B; ++k) sl.h_pcs[k] = request_consts(S, sl.reqs[k]->r);
   const PointConsts& pc0 = sl.h_pcs[0];
-  const double cmax = max_cutoff(pc0, S->L->ls.max_nu, d);
+  const double cmax = batch_cutmax(S, sl.reqs);
   size_t lo, hi;
   line_slice(S->L, r0.start_nu, r0.start_nu + M * d, cmax, d, lo, hi);
   const int n = (int)(hi - lo);
@@ -341,18 +356,26 @@ void scheduler(gx_server* S) {
         // HBM admission: footprint of the other slots plus this slot after growth
         const gx_request& r = head->r;
         const double d = (r.end_nu - r.start_nu) / double(r.npoints);
-        gx_consts c{};
-        c.temperature = r.temperature; c.partition = r.partition; c.dfreq = d; c.mean_mass = S->o.mean_mass;
-        c.pressure = r.pressure; c.ref_temp = S->o.ref_temp; c.cutoff = r.cutoff; c.doppler_widths = r.doppler_widths;
-        const PointConsts pc = make_point_consts(r.profile, c, r.pressure);
         size_t lo, hi;
-        line_slice(S->L, r.start_nu, r.end_nu, max_cutoff(pc, S->L->ls.max_nu, d), d, lo, hi);
+        line_slice(S->L, r.start_nu, r.end_nu, batch_cutmax(S, batch), d, lo, hi);
         size_t others = 0;
         for (auto& o2 : S->slots) if (o2.get() != &sl) others += o2->footprint();
         size_t need = batch_need(S, sl, batch, hi - lo);
         while (batch.size() > 1 && others + need > S->budget) {   // shrink the batch first
           batch.pop_back();
           need = batch_need(S, sl, batch, hi - lo);
+        }
+        if (others + need > S->budget) {
+          // idle slots keep their grown buffers: let them go, largest first, before waiting
+          std::vector<Slot*> idle;
+          for (auto& o2 : S->slots)
+            if (o2.get() != &sl && !o2->busy && o2->footprint() > 0) idle.push_back(o2.get());
+          std::sort(idle.begin(), idle.end(), [](Slot* a, Slot* b) { return a->footprint() > b->footprint(); });
+          for (Slot* o2 : idle) {
+            if (others + need <= S->budget) break;
+            others -= o2->footprint();
+            o2->release_buffers();
+          }
         }
         if (others + need > S->budget) {
           bool any_busy = false;
@@ -512,7 +535,15 @@ int gx_server_result(gx_server* S, uint64_t id, double* out, int n, gx_timing* t
   auto it = S->reqs.find(id);
   if (it == S->reqs.end()) return fail(GX_NOT_FOUND, "server_result: unknown request");
   Req* q = it->second.get();
-  if (q->state == GX_REQ_FAILED) return fail(GX_DEVICE_ERROR, "server_result: request failed: " + q->error);
+  if (q->state == GX_REQ_FAILED) {
+    const std::string msg = "server_result: request failed: " + q->error;
+    S->reqs.erase(it);   // released: a long-running server keeps no record of it
+    return fail(GX_DEVICE_ERROR, msg);
+  }
+  if (q->state == GX_REQ_CANCELLED) {
+    S->reqs.erase(it);
+    return fail(GX_NOT_FOUND, "server_result: request was cancelled");
+  }
   if (q->state != GX_REQ_DONE) return fail(GX_BUSY, "server_result: request not finished");
   if (n != q->r.npoints) return fail(GX_INVALID_ARG, "server_result: n differs from the request's npoints");
   std::copy(q->host.begin(), q->host.end(), out);

@@ -38,6 +38,7 @@ namespace {
 
 struct Group {                // points that share one cutoff (hence windows and block ranges)
   double cutoff = 0;
+  double cutmax = 0;          // widest per-line window of its points (Doppler k*alpha + dfreq can exceed cutoff)
   std::vector<int> points;
   int first_const = 0;        // index of points[0] in the grouped constants table
   int* ib = nullptr;
@@ -113,12 +114,13 @@ int enqueue(gx_table* t, bool timed) {
   if ((e = launch_fill(t->d_out, (size_t)t->M * t->pts.size(), 0.0, s))) return hip_fail(e, "fill");
   if (t->core_evals && (e = launch_fill(reinterpret_cast<double*>(t->core_evals), kEvalSlots, 0.0, s)))
     return hip_fail(e, "fill");
+  const bool mixed = mixed_table(t);
   for (Group& g : t->groups) {
     if ((e = launch_fill(reinterpret_cast<double*>(g.evals), kEvalSlots, 0.0, s))) return hip_fail(e, "fill");
+    if (!mixed) continue;   // the FP64 path's per-point prep forms (and counts) each point's own windows
     const WindowArgs wa{g.cutoff, t->o.convention, 1};
     if ((e = launch_windows(wa, gv, lv.nu, lv.n, g.ib, g.ie, g.evals, s))) return hip_fail(e, "windows");
   }
-  const bool mixed = mixed_table(t);
   if (mixed) {
     for (Group& g : t->groups) {
       const double cut = g.cutoff;
@@ -140,10 +142,10 @@ int enqueue(gx_table* t, bool timed) {
   } else {
     if (timed && (e = hipEventRecord(t->ev[1], s))) return hip_fail(e, "event");
     for (Group& g : t->groups) {
-      if ((e = launch_block_ranges(gv, lv.nu, lv.n, g.cutoff, kBinsF64, g.nblocks64, g.rng64.r, s)))
+      if ((e = launch_block_ranges(gv, lv.nu, lv.n, g.cutmax, kBinsF64, g.nblocks64, g.rng64.r, s)))
         return hip_fail(e, "ranges");
       for (int p : g.points) {
-        if ((e = launch_line_prep_f64(t->pcs[p], gv, lv, t->der64.d, nullptr, s))) return hip_fail(e, "prep");
+        if ((e = launch_line_prep_f64(t->pcs[p], gv, lv, t->der64.d, g.evals, s))) return hip_fail(e, "prep");
         if ((e = launch_accum_f64(t->pcs[p], gv, lv.nu, t->der64.d, g.rng64.r, g.nblocks64, t->d_out + (size_t)p * t->M,
                                   s)))
           return hip_fail(e, "accumulate");
@@ -404,6 +406,7 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
       c.precision = o->precision;
       t->pcs[p] = make_point_consts(o->profile, c, pts[p].pressure);
       grouped.push_back(t->pcs[p]);
+      g.cutmax = std::max(g.cutmax, max_cutoff(t->pcs[p], L->ls.max_nu, t->dfreq));
     }
   }
   // a grid too fine for the mixed path's W4 core pass at some point runs as an FP64 table
@@ -423,9 +426,13 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
     size_t free_b = 0, total_b = 0;
     if ((e = hipMemGetInfo(&free_b, &total_b))) return bail(hip_fail(e, "hipMemGetInfo"));
     const size_t headroom = size_t(2) << 30;
-    // amp, y, 1/alpha, core range per line, the core pass's rows (~3 doubles per bin) and
-    // the far-field moments and block plans
-    const size_t per_point = 28 * std::max<size_t>(n, 1) + 32 * (size_t)t->M + (size_t(64) << 10) +
+    // amp, y, 1/alpha, core range per line, the core pass's rows (from the same geometry the
+    // launch uses, at the point with the widest cores) and the far-field moments and block
+    // plans; segment partials exist only while a launch has fewer items than ~4 per resident
+    // wave, so they stay below 2 x 4 x 6144 items x 512 bins x 8 B = 200 MB, inside the headroom
+    size_t rows = 0;
+    for (int p = 0; p < npts; ++p) rows = std::max(rows, core_rows_bytes(grid_view(t), t->pcs[p].dpwcoef));
+    const size_t per_point = 28 * std::max<size_t>(n, 1) + rows + (size_t(64) << 10) +
                              far_field_bytes(1, (long long)std::max<size_t>(n, 1), t->M / 64 + 1);
     const size_t avail = free_b > headroom ? free_b - headroom : 0;
     t->batch = (int)std::min<size_t>((size_t)npts, avail / per_point);
@@ -512,7 +519,8 @@ int gx_table_wait(gx_table* t, gx_timing* timing) {
     if ((e = hipMemcpy(slots, g.evals, sizeof(slots), hipMemcpyDeviceToHost))) return hip_fail(e, "table_wait: copy");
     unsigned long long s = 0;
     for (auto v : slots) s += v;
-    evals += (long long)s * (long long)g.points.size();
+    // mixed: one shared window set per group; FP64: every point counted its own
+    evals += mixed_table(t) ? (long long)s * (long long)g.points.size() : (long long)s;
   }
   timing->intensity_ms = a;
   timing->accumulate_ms = b;

@@ -76,6 +76,11 @@ CASES = [
     dict(name="doppler_T1500_t2_chunked", src=("doppler", 13, 100_000), sel=(100.0, 400.0, "sorted"),
          profile=1, T=1500.0, P=1.0, ref_temp=296.0, mass=30.01, start=200.0, end=300.0, npoints=12000,
          nthreads=2, capacity=700, max_points=80000),
+    # the 1e7-line Voigt workload's own grid (0-12000 cm^-1 at 0.01 cm^-1, 1.2e6 bins), T = 1000 K,
+    # P = 1 atm, its state count and energy ranges; a thinned line list around 3000-3100 cm^-1
+    dict(name="voigt_T1000_P1_grid1e7", src=("voigt", 2, 400_000, 200_000), sel=(2990.0, 3110.0, "sorted"),
+         profile=2, T=1000.0, P=1.0, ref_temp=500.0, mass=17.03, start=0.0, end=12000.0, npoints=1_200_000,
+         nthreads=4, capacity=1 << 30, max_points=80000),
     dict(name="voigt_halfbin_ties", src=("ties",), sel=None,
          profile=2, T=700.0, P=1.0, ref_temp=500.0, mass=17.03, start=0.0, end=128.0, npoints=16384,
          nthreads=2, capacity=1 << 30, max_points=80000),
@@ -87,9 +92,10 @@ def build_case(case):
         lines = tie_lines(case)
         case["Q"] = 1234.5
     else:
-        kind, seed, n = case["src"]
+        kind, seed, n = case["src"][:3]
+        n_states = case["src"][3] if len(case["src"]) > 3 else 20_000
         voigt = kind == "voigt"
-        ll = make_lines(seed, 20_000, 20000.0 if voigt else 15000.0, n, 12000.0 if voigt else 10000.0,
+        ll = make_lines(seed, n_states, 20000.0 if voigt else 15000.0, n, 12000.0 if voigt else 10000.0,
                         case["mass"], voigt=voigt)
         idx = select(ll, *case["sel"])
         lines = dict(elow=ll.elow[idx], nu=ll.nu[idx], aif=ll.aif[idx], gns=ll.gns[idx])
@@ -101,11 +107,18 @@ def build_case(case):
     return lines
 
 
-def main():
+def main(only=None):
+    """Write every fixture (or only the named ones) and the manifest of all fixtures."""
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "liboracle.so"], check=True)
     manifest = []
     for case in CASES:
         case = dict(case)
+        if only and case["name"] not in only:
+            z = np.load(os.path.join(HERE, case["name"] + ".npz"))
+            out, nl = z["intens"], int(z["nu"].size)
+            manifest.append(dict(name=case["name"], nlines=nl, npoints=case["npoints"], max=float(out.max()),
+                                 nonzero=int(np.count_nonzero(out))))
+            continue
         lines = build_case(case)
         out = run_reference(case, lines)
         meta = {k: v for k, v in case.items() if k not in ("src", "sel")}
@@ -119,4 +132,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or None)

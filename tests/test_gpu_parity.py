@@ -124,6 +124,34 @@ def test_doppler_fp64_vs_oracle(T, widths):
     assert rel_err(out, ref) <= FP64_TOL
 
 
+@pytest.mark.parametrize("T,widths", [(296.0, 0.0), (1500.0, 0.0), (296.0, 25.0), (2500.0, 25.0)])
+def test_doppler_gpu_convention_vs_oracle(T, widths):
+    """The CUDA manager's Doppler conventions (cross_kernal.cu:645-647, 667, 686-687): the 1/e
+    width, a line at nu = 0 adds nothing (S = 0), and bin j takes a line iff
+    -C <= nu - f_j <= C.  Lines at nu = 0 and within the window of a grid starting at 0."""
+    ll = make_lines(1, 20000, 15000.0, 200_000, 10000.0, 30.01, voigt=False)
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero(ll.nu < 220.0)[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(0.0, 100.0, 10000)
+    # two lines at nu = 0 and one exactly C above a node (the window's inclusive edge)
+    extra = np.array([0.0, 0.0, freq[4000] + 100.0])
+    L.nu = np.concatenate([L.nu, extra])
+    L.elow = np.concatenate([L.elow, [100.0, 2000.0, 500.0]])
+    L.aif = np.concatenate([L.aif, [1.0, 10.0, 5.0]])
+    L.gns = np.concatenate([L.gns, np.array([3, 5, 7], np.int32)])
+    print(f"doppler gpu convention T={T} widths={widths} lines={L.n}", flush=True)
+    out, t = run_manager(capi.DOPPLER, L, freq, d, T, Q, 30.01, 1.0, ref_temp=296.0, doppler_widths=widths,
+                         convention=capi.CONV_GPU)
+    c = po.consts(po.DOPPLER, T, Q, d, 30.01, 1.0, 296.0, convention=po.CONV_GPU, doppler_widths=widths)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, None, None, freq, freq.size, 0, 4)
+    assert np.isfinite(out).all()
+    assert rel_err(out, ref) <= FP64_TOL
+    # lines with S = 0 add no evaluations on the device; every other window as the oracle's
+    nz = L.nu != 0.0
+    assert t.evals == po.count_evals(c, L.nu[nz], freq, freq.size, 0)
+
+
 @pytest.mark.parametrize("T,P", [(1500.0, 0.1), (300.0, 1e-3), (3000.0, 10.0), (1000.0, 1.0)])
 def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
     ll = voigt_lines

@@ -98,3 +98,84 @@ def test_cancel_and_budget_rejection(world):
         assert srv.wait(rid, 60.0) == capi.REQ_FAILED
         with pytest.raises(capi.GxError):
             srv.result(rid)
+
+
+def test_doppler_batch_of_two_temperatures_near_the_top(world):
+    """Per-line Doppler windows (k alpha + dfreq) widen with temperature: a batch of a cold
+    and a hot request on one grid near the top of the list takes the hot one's line slice,
+    so each equals its own run alone bit for bit, and the oracle within 1e-10."""
+    ll, _ = world
+    rng = np.random.default_rng(11)
+    n = 40_000                                   # 40 lines per cm^-1 in [11000, 12000]
+    nu = np.sort(rng.uniform(11000.0, 12000.0, n))
+    elow = rng.uniform(0.0, 8000.0, n)
+    aif = 10 ** rng.uniform(-6, 2, n)
+    gns = rng.integers(1, 40, n).astype(np.int32)
+    lines = ResidentLines(elow, nu, aif, gns)
+    grid = (11500.0, 11950.0, 45_000)
+    reqs = [dict(T=T, P=1.0, Q=po.partition(ll.state_e, ll.state_g, T), grid=grid, profile=capi.DOPPLER,
+                 precision=capi.PREC_FP64, cutoff=0.0, doppler_widths=3.0) for T in (300.0, 3000.0)]
+    try:
+        with Server(lines, 17.03, max_inflight=1, max_batch=4) as srv:
+            ids = [submit(srv, r) for r in reqs]
+            mix = []
+            for rid in ids:
+                assert srv.wait(rid, 120.0) == capi.REQ_DONE
+                mix.append(srv.result(rid)[0])
+            st = srv.stats()
+            print(f"doppler batch: {st.batches} batches for {st.completed} requests", flush=True)
+            assert st.batches == 1                 # both in one launch
+        with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
+            for r, m in zip(reqs, mix):
+                rid = submit(srv, r)
+                assert srv.wait(rid, 120.0) == capi.REQ_DONE
+                assert np.array_equal(srv.result(rid)[0], m)
+    finally:
+        lines.close()
+    start, end, npts = grid
+    freq, d = make_grid(start, end, npts)
+    for r, m in zip(reqs, mix):
+        c = po.consts(capi.DOPPLER, r["T"], r["Q"], d, 17.03, 1.0, 500.0, cutoff=100.0, doppler_widths=3.0)
+        ref = po.cross_section(c, elow, nu, aif, gns, None, None, freq, npts, 0, 8)
+        mask = ref >= 1e-6 * ref.max()
+        err = float(np.max(np.abs(m[mask] - ref[mask]) / ref[mask]))
+        print(f"  T={r['T']}: max rel err {err:.2e}", flush=True)
+        assert err <= 1e-10
+        assert np.all(m[ref > 1e-9 * ref.max()] > 0)   # edge bins reached by the hot lines included
+
+
+def test_tight_budget_lets_idle_slots_release(world):
+    """With a budget for one large request and one small one, alternating large and small
+    requests all finish: idle slots give back their grown buffers before admission waits."""
+    ll, lines = world
+    Q = po.partition(ll.state_e, ll.state_g, 1200.0)
+    large = (500.0, 9500.0, 900_000)
+    small = (3000.0, 3010.0, 1000)
+    sub = lambda srv, g: srv.submit(1200.0, 0.5, Q, *g, profile=capi.VOIGT, precision=capi.PREC_FP64, cutoff=25.0)
+    with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
+        rid = sub(srv, large)
+        assert srv.wait(rid, 120.0) == capi.REQ_DONE
+        need = srv.stats().peak_bytes
+    budget = need + need // 2
+    print(f"budget {budget} B for a large request of {need} B", flush=True)
+    with Server(lines, 17.03, max_inflight=2, max_batch=1, hbm_budget=budget) as srv:
+        ids = [sub(srv, large if k % 2 == 0 else small) for k in range(6)]
+        states = [srv.wait(rid, 120.0) for rid in ids]
+        st = srv.stats()
+        assert states == [capi.REQ_DONE] * 6, states
+        assert st.failed == 0
+        assert st.peak_bytes <= budget, (st.peak_bytes, budget)
+
+
+def test_failed_and_cancelled_requests_are_released(world):
+    ll, lines = world
+    Q = po.partition(ll.state_e, ll.state_g, 1000.0)
+    with Server(lines, 17.03, hbm_budget=1 << 20) as srv:        # 1 MiB: nothing fits
+        rid = srv.submit(1000.0, 1.0, Q, 100.0, 11000.0, 200_000)
+        assert srv.wait(rid, 60.0) == capi.REQ_FAILED
+        with pytest.raises(capi.GxError) as ei:
+            srv.result(rid)
+        assert ei.value.status == capi.GX_DEVICE_ERROR
+        with pytest.raises(capi.GxError) as ei:           # released: the id is gone
+            srv.poll(rid)
+        assert ei.value.status == capi.GX_NOT_FOUND

@@ -139,3 +139,35 @@ def test_table_saved_to_an_opacity_file(setup, tmp_path):
         assert np.array_equal(np.asarray(t.grids[k]), tab.read(k))
     freq, _ = make_grid(2050.0, 2150.0, M)
     assert np.array_equal(t.freq, freq)
+
+
+def test_mixed_table_at_the_48_point_set_vs_oracle():
+    """The 48-point opacity-table workload's own (T,P) set -- 8 temperatures log-spaced
+    300-3000 K x 6 pressures log-spaced 1e-4..100 atm, 25 cm^-1 cutoff, 0.01 cm^-1 bins --
+    as ONE graph-captured mixed-precision table on a 200 cm^-1 sub-range: every grid within
+    1e-5 of the FP64 oracle, the far field engaged (real and complex-centred moments)."""
+    from gpu_cross_amd.synth import WORKLOADS
+    wl = WORKLOADS["table_48"]
+    ll = make_lines(41, 40000, 20000.0, 600_000, 12000.0, 17.03)
+    sel = np.nonzero((ll.nu > 5970.0) & (ll.nu < 6230.0))[0]
+    sub = ll.subset(sel)
+    lines = ResidentLines.from_linelist(sub)
+    pts = [(T, P, po.partition(ll.state_e, ll.state_g, T)) for T in wl["temperatures"] for P in wl["pressures"]]
+    assert len(pts) == 48 and pts[0][1] == 1e-4 and abs(pts[5][1] - 100.0) < 1e-9
+    M = 20000
+    freq, d = make_grid(6000.0, 6200.0, M)
+    assert d == 0.01
+    print(f"48-point table: {sub.n} lines, {M} bins", flush=True)
+    tab = Table(lines, 6000.0, 6200.0, M, pts, 17.03, cutoff=25.0, use_graph=True, autotune=True)
+    tab.run()
+    t = tab.wait()
+    print(f"  far field {t.far_evals / t.evals:.1%} of pairs, core pass {t.core_evals / t.evals:.2%}", flush=True)
+    assert t.far_evals > 0 and t.core_evals > 0
+    worst = 0.0
+    for p, (T, P, Q) in enumerate(pts):
+        err = rel_err(tab.read(p), oracle_grid(sub, T, P, Q, freq, d, 25.0))
+        worst = max(worst, err)
+        assert err <= 1e-5, (T, P, err)
+    print(f"  worst grid: max rel err {worst:.2e}", flush=True)
+    tab.close()
+    lines.close()
