@@ -18,7 +18,7 @@ HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", h)
                       for h in ("gx_ingest.h", "gx_output.h", "gx_hitran.h")]
 
 GX_OK, GX_INVALID_ARG, GX_OUT_OF_MEMORY, GX_NO_DEVICE, GX_DEVICE_ERROR, GX_BUSY, GX_NOT_FOUND = range(7)
-DOPPLER, VOIGT = 1, 2
+GAUSSIAN, DOPPLER, VOIGT = 0, 1, 2
 CONV_OPENMP, CONV_GPU = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 

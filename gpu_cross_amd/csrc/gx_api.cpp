@@ -47,6 +47,30 @@ constexpr double kNA = 6.0221412927e23, kLn2 = 0.69314718056, kPi = 3.1415926535
 
 PointConsts make_point_consts(int profile, const gx_consts& c, double pressure_scale) {
   PointConsts p{};
+  if (profile == kProfGaussian || profile == kProfVoigtNoGamma) {
+    // sums the reference has only in its CUDA manager: its constants and per-bin windows,
+    // whatever the convention (GPUManager.cpp:24-47, cross_kernal.cu:471-625, 731-922)
+    p.profile = profile;
+    p.convention = GX_CONV_GPU;
+    p.temperature = c.temperature;
+    p.partition = c.partition;
+    p.ref_temp = c.ref_temp;
+    p.beta = kPlanck * kVellgt / (kBoltz * c.temperature);
+    p.cmcoef = 1.0 / (8.0 * std::acos(-1.0) * 2.99792458e10);
+    if (profile == kProfGaussian) {
+      // x_ = sqrt(ln2)/HW with ln2 the defines.h:4 literal: the reference's cross_constants.ln2
+      // is never set (GPUManager.cpp:24-47), so its value there is undefined
+      p.dpwcoef = std::sqrt(kLn2) / c.half_width;
+      p.cutoff = c.cutoff > 0.0 ? c.cutoff : c.half_width * 10.0;          // cross_kernal.cu:578
+    } else {
+      // cross_kernal.cu:872 as written: no Avogadro factor, so with the mean mass in amu the
+      // Doppler width is ~1e-16 cm^-1 (every bin then takes the 3-point form)
+      p.dpwcoef = std::sqrt(2.0 * kLn2 * kBoltz * c.temperature / (c.mean_mass)) / kVellgt;
+      p.cutoff = c.cutoff > 0.0 ? c.cutoff : 500.0 * 0.05;                 // :885-888
+    }
+    p.pressure_scale = pressure_scale;
+    return p;
+  }
   p.profile = profile;
   p.convention = c.convention;
   p.temperature = c.temperature;
@@ -110,6 +134,7 @@ int LineStore::upload(size_t nlines, const double* h_e, const double* h_nu, cons
   const bool voigt = h_gam != nullptr;
   int rc = reserve(nlines, voigt);
   if (rc) return rc;
+  has_gamma = voigt;
   bool sorted = true;
   double mx = 0.0;
   for (size_t i = 0; i < nlines; ++i) {
@@ -644,6 +669,13 @@ int set_device(gx_ctx* c) {
   return e ? hip_fail(e, "hipSetDevice") : GX_OK;
 }
 
+// The sum a chunk takes: a Voigt chunk uploaded without gamma and n takes the CUDA manager's
+// no-gamma sum (GPUManager.cpp:139-143 chooses it by the gamma buffer being absent).
+int sum_profile(const gx_ctx* c) {
+  if (c->profile == GX_VOIGT && c->lines.n > 0 && !c->lines.has_gamma) return kProfVoigtNoGamma;
+  return c->profile;
+}
+
 }  // namespace
 
 extern "C" {
@@ -651,8 +683,8 @@ extern "C" {
 int gx_create(int device, int profile, size_t mem_budget, gx_ctx** out) {
   if (!out) return fail(GX_INVALID_ARG, "gx_create: out is NULL");
   *out = nullptr;
-  if (profile != GX_DOPPLER && profile != GX_VOIGT)
-    return fail(GX_INVALID_ARG, "gx_create: only DOPPLER (1) and VOIGT (2) profiles are implemented");
+  if (profile != GX_GAUSSIAN && profile != GX_DOPPLER && profile != GX_VOIGT)
+    return fail(GX_INVALID_ARG, "gx_create: only the GAUSSIAN (0), DOPPLER (1) and VOIGT (2) profiles are implemented");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e || ndev <= 0) return fail(GX_NO_DEVICE, "gx_create: no HIP device");
@@ -709,6 +741,8 @@ int gx_initialize_constants(gx_ctx* c, const gx_consts* k) {
     return fail(GX_INVALID_ARG, "initialize_constants: unknown convention");
   if (k->precision != GX_PREC_FP64 && k->precision != GX_PREC_MIXED)
     return fail(GX_INVALID_ARG, "initialize_constants: unknown precision");
+  if (c->profile == GX_GAUSSIAN && !(k->half_width > 0))
+    return fail(GX_INVALID_ARG, "initialize_constants: the Gaussian profile needs half_width > 0");
   c->consts = *k;
   c->have_consts = true;
   return GX_OK;
@@ -737,8 +771,8 @@ int gx_transfer_vectors(gx_ctx* c, size_t nener, const double* energies, const d
   if (!c) return fail(GX_INVALID_ARG, "transfer_vectors: NULL context");
   if (nener > 0 && (!energies || !nu || !aif || !gns))
     return fail(GX_INVALID_ARG, "transfer_vectors: NULL line array");
-  if (c->profile == GX_VOIGT && nener > 0 && (!gamma || !nexp))
-    return fail(GX_INVALID_ARG, "transfer_vectors: Voigt needs gamma and n");
+  if (c->profile == GX_VOIGT && nener > 0 && (!gamma != !nexp))
+    return fail(GX_INVALID_ARG, "transfer_vectors: Voigt takes gamma and n together (or neither)");
   if (nener > c->ntrans && c->ntrans) return fail(GX_INVALID_ARG, "transfer_vectors: more lines than GetNtrans()");
   if (nener > (size_t)INT32_MAX) return fail(GX_INVALID_ARG, "transfer_vectors: chunk too large");
   int rc = set_device(c);
@@ -759,13 +793,13 @@ int gx_execute(gx_ctx* c, int points, int n_ener, int start_idx) {
   int rc = set_device(c);
   if (rc) return rc;
   const gx_consts& k = c->consts;
-  PointConsts pc = make_point_consts(c->profile, k, 1.0);
+  PointConsts pc = make_point_consts(sum_profile(c), k, 1.0);
   GridView gv{c->d_freq, c->M, start_idx, points, c->start_nu, k.dfreq};
   hipError_t e = hipMemsetAsync(c->d_evals, 0, sizeof(unsigned long long) * kEvalSlots, c->stream);
   if (e) return hip_fail(e, "execute: memset");
   c->last_lines = n_ener;
   c->timed = true;
-  c->last_mixed = k.precision == GX_PREC_MIXED && c->profile == GX_VOIGT && core_pass_fits(gv, pc.dpwcoef);
+  c->last_mixed = k.precision == GX_PREC_MIXED && pc.profile == GX_VOIGT && core_pass_fits(gv, pc.dpwcoef);
   return run_point(pc, gv, c->lines, n_ener, k.precision, c->derived, c->ranges, c->mix, c->d_intens, c->d_evals,
                    c->d_core_evals, c->stream, c->ev);
 }
@@ -784,7 +818,7 @@ int gx_execute_noalloc(gx_ctx* c, int points, int n_ener, int start_idx) {
   int rc = set_device(c);
   if (rc) return rc;
   const gx_consts& k = c->consts;
-  PointConsts pc = make_point_consts(c->profile, k, 1.0);
+  PointConsts pc = make_point_consts(sum_profile(c), k, 1.0);
   GridView gv{c->d_freq, c->M, start_idx, points, c->start_nu, k.dfreq};
   c->timed = false;   // gx_last_timing covers gx_execute only
   return execute_noalloc(c->device, pc, gv, c->lines, n_ener, c->d_intens, c->stream);

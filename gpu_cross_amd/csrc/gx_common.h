@@ -33,6 +33,7 @@ struct LineStore {
   size_t n = 0;
   size_t cap = 0;
   double max_nu = 0.0;
+  bool has_gamma = false;   // the last upload carried per-line gamma and n
   int reserve(size_t n, bool voigt);
   void release();
   int upload(size_t n, const double* e, const double* nu, const double* a, const int* g, const double* gam,

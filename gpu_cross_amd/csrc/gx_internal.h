@@ -21,9 +21,19 @@ struct PointConsts {
   double pressure_scale;  // multiplies gamma (1 when gamma already holds gamma*P)
   double cutoff;          // window half-width, cm^-1
   double doppler_widths;  // > 0: per-line Doppler window k*alpha + dfreq
-  int profile;            // 1 Doppler, 2 Voigt
+  int profile;            // kProfGaussian, kProfDoppler, kProfVoigt or kProfVoigtNoGamma
   int convention;         // 0 openmp, 1 gpu
 };
+
+// Profiles of PointConsts: the reference's ProfileType numbering (Input.h:9-15) for the first
+// three, and the Voigt sum of a line list uploaded without per-line Lorentz widths (the CUDA
+// manager's cross_kernal.cu:860-922), which has no number of its own there.  For the Gaussian,
+// dpwcoef holds x_ = sqrt(ln2)/HW; for the no-gamma Voigt, the Doppler width per unit nu as
+// cross_kernal.cu:872 forms it.
+constexpr int kProfGaussian = 0;
+constexpr int kProfDoppler = 1;
+constexpr int kProfVoigt = 2;
+constexpr int kProfVoigtNoGamma = 4;
 
 // Grid and chunk geometry.
 struct GridView {

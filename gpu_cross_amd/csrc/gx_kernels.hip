@@ -18,6 +18,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "gx_internal.h"
+#include "gx_plain.h"
 #include "gx_w4.h"
 
 namespace gx {
@@ -103,7 +104,33 @@ __global__ __launch_bounds__(256) void k_line_prep(PointConsts pc, GridView gv, 
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long w = 0;
-  if (i < lv.n) {
+  if (i < lv.n && (pc.profile == kProfGaussian || pc.profile == kProfVoigtNoGamma)) {
+    // the CUDA manager's intensity step of both (cross_kernal.cu:471-496, 731-756): its cmcoef,
+    // no ISQRTPI, nu == 0 gives S = 0; per-bin windows |nu - f| <= C
+    const double nu = lv.nu[i];
+    const double a = lv.aif[i];
+    const int gi = lv.gns[i];
+    const double e = lv.elow[i];
+    double S = pc.cmcoef * a * gi * exp(-pc.beta * e) * (1.0 - exp(-pc.beta * nu)) / (nu * nu * pc.partition);
+    if (nu == 0) S = 0.0;
+    double g, y = 0.0;
+    if (pc.profile == kProfGaussian) {
+      g = pc.dpwcoef;                              // x_ = sqrt(ln2) / HW (cross_kernal.cu:582)
+    } else {
+      g = nu * pc.dpwcoef;                         // gamma_G (cross_kernal.cu:885)
+      y = kPlainSqrtLn2 * kNoGammaL / g;           // :887
+    }
+    const int s0 = gv.start_idx, s1 = gv.start_idx + gv.points;
+    const int ib = first_le(nu, pc.cutoff, gv.freq, s0, s1);
+    int ie = first_lt(nu, -pc.cutoff, gv.freq, s0, s1);
+    if (S == 0.0 || ie < ib) ie = ib;
+    out.S[i] = S;
+    out.g[i] = g;
+    out.y[i] = y;
+    out.ib[i] = ib;
+    out.ie[i] = ie;
+    w = (unsigned long long)(ie - ib);
+  } else if (i < lv.n) {
     const bool voigt = pc.profile == 2;
     const bool gpu = pc.convention == 1;
     const double nu = lv.nu[i];
@@ -333,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
       s_nu[t] = v == 0.0 ? zero_nu : v;
       s_S[t] = S[i];
       s_g[t] = G[i];
-      if (PROFILE == 2) s_y[t] = Y[i];
+      if (PROFILE == kProfVoigt || PROFILE == kProfVoigtNoGamma) s_y[t] = Y[i];
       s_ib[t] = IB[i];
       s_ie[t] = IE[i];
     }
@@ -343,9 +370,13 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
       const int ib = s_ib[k], ie = s_ie[k];
       if (j >= ib && j < ie) {
         const double g = s_g[k];
-        if (PROFILE == 2) {
+        if (PROFILE == kProfVoigt) {
           const double x = fabs(s_nu[k] - f) * g;  // OpenMPManager.cpp:211-214
           acc += s_S[k] * w4(x, s_y[k]) * g;
+        } else if (PROFILE == kProfGaussian) {
+          acc += plain_gauss_term(s_S[k], f - s_nu[k], g, x0);       // cross_kernal.cu:594-604
+        } else if (PROFILE == kProfVoigtNoGamma) {
+          acc += plain_nogamma_term(s_S[k], s_nu[k] - f, g, s_y[k]);  // cross_kernal.cu:884-904
         } else {
           const double d = fabs(s_nu[k] - f);      // OpenMPManager.cpp:165-171
           const double xm = g * (d - x0);
@@ -908,9 +939,15 @@ hipError_t launch_block_ranges(const GridView& gv, const double* nu, int n, doub
 hipError_t launch_accum_f64(const PointConsts& pc, const GridView& gv, const double* nu_sorted, DerivedF64 d,
                             const int2* ranges, int nblocks, double* intens, hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
-  const bool zero_to_tiny = pc.profile == 2 || pc.convention == 0;
+  const bool zero_to_tiny = pc.profile == kProfVoigt || (pc.profile == kProfDoppler && pc.convention == 0);
   const double zero_nu = zero_to_tiny ? 1e-6 : 0.0;
-  if (pc.profile == 2)
+  if (pc.profile == kProfGaussian)
+    k_accum_f64<kProfGaussian><<<nblocks, kBlock, 0, s>>>(gv, nu_sorted, d.S, d.g, d.y, d.ib, d.ie, ranges, nblocks,
+                                                          intens, zero_nu, pc.dpwcoef * gv.dfreq * 0.5);
+  else if (pc.profile == kProfVoigtNoGamma)
+    k_accum_f64<kProfVoigtNoGamma><<<nblocks, kBlock, 0, s>>>(gv, nu_sorted, d.S, d.g, d.y, d.ib, d.ie, ranges,
+                                                              nblocks, intens, zero_nu, 0.0);
+  else if (pc.profile == 2)
     k_accum_f64<2><<<nblocks, kBlock, 0, s>>>(gv, nu_sorted, d.S, d.g, d.y, d.ib, d.ie, ranges, nblocks, intens,
                                               zero_nu, 0.0);
   else

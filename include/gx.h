@@ -34,8 +34,12 @@ typedef enum gx_status {
     GX_NOT_FOUND = 6
 } gx_status;
 
-/* Input.h:9-15 ProfileType numbering (GAUSSIAN=0 and PSUEDO_VOIGT=3 are not served) */
-enum { GX_DOPPLER = 1, GX_VOIGT = 2 };
+/* Input.h:9-15 ProfileType numbering (PSUEDO_VOIGT=3 is not served).  GX_GAUSSIAN is the
+ * CUDA manager's fixed-half-width profile (cross_kernal.cu:471-625): gx_consts.half_width is
+ * the HWHM, window 10 HWHM, FP64, manager entry points only.  A GX_VOIGT chunk uploaded
+ * without gamma and n takes the CUDA manager's sum for that case (cross_kernal.cu:860-922:
+ * gamma_L = 0.05, Alg. 916 / 3-point Gauss-Hermite profile, FP64). */
+enum { GX_GAUSSIAN = 0, GX_DOPPLER = 1, GX_VOIGT = 2 };
 /* Width / window / constant convention: the CPU manager's (OpenMPManager.cpp:146,191,205-206)
  * or the CUDA manager's (cross_kernal.cu:667,938,954-957, GPUManager.cpp:31) */
 enum { GX_CONV_OPENMP = 0, GX_CONV_GPU = 1 };
@@ -49,7 +53,7 @@ typedef struct gx_ctx gx_ctx;
 /* Arguments of BaseManager::InitializeConstants (BaseManager.h:23) plus the
  * knobs the reference hard-codes. */
 typedef struct gx_consts {
-    double half_width;      /* unused by Doppler/Voigt, kept for signature parity */
+    double half_width;      /* Gaussian HWHM in cm^-1 (unused by Doppler/Voigt) */
     double temperature;     /* K */
     double partition;       /* Q(T) */
     double dfreq;           /* bin width of the grid passed to gx_transfer_freq */
@@ -100,8 +104,9 @@ int gx_initialize_constants(gx_ctx *ctx, const gx_consts *consts);
 int gx_transfer_freq(gx_ctx *ctx, const double *freq, const double *intens, int n);
 
 /* TransferVectors(Nener, E'', nu, A, g, gamma, n) (GPUManager.cpp:87-105):
- * waits for prior work, copies the chunk (gamma/nexp NULL for Doppler), and
- * sorts it by nu on the device when it is not already sorted.  Blocking. */
+ * waits for prior work, copies the chunk (gamma/nexp NULL for Doppler and Gaussian;
+ * both NULL for a Voigt chunk selects the no-gamma sum), and sorts it by nu on the
+ * device when it is not already sorted.  Blocking. */
 int gx_transfer_vectors(gx_ctx *ctx, size_t nener, const double *energies, const double *nu, const double *aif,
                         const int *gns, const double *gamma, const double *nexp);
 

@@ -123,6 +123,82 @@ double orc_line_intensity(const orc_consts *c, double elow, double nu, double ai
     return s;
 }
 
+double orc_line_intensity_plain(const orc_consts *c, double elow, double nu, double aif, int gns)
+{
+    /* cross_kernal.cu:471-496 (Gaussian) and :731-756 (Voigt without gamma): the CUDA manager's
+     * cmcoef (acos(-1), GPUManager.cpp:31,39), no ISQRTPI, nu == 0 gives S = 0 */
+    const double beta = orc_beta(c->temperature);
+    const double cm = orc_cmcoef(ORC_CONV_GPU);
+    const double g = (double)gns;
+    double s = cm * aif * g * exp(-beta * elow) * (1.0 - exp(-beta * nu)) / (nu * nu * c->partition);
+    return nu == 0 ? 0.0 : s;
+}
+
+/* erfcx(y) = exp(y^2) erfc(y) for y >= 0.  Below 26: y^2 = h + l exactly (Dekker's product
+ * with fma), exp(y^2) = exp(h) (1 + l) to first order in |l| <= ulp(h)/2, times libm erfc
+ * (erfc(26) ~ 5e-296 is still a normal number).  From 26 up: Laplace's continued fraction
+ * sqrt(pi) erfcx(y) = 1/(y + (1/2)/(y + 1/(y + (3/2)/(y + 2/(y + ...))))) (Abramowitz &
+ * Stegun 7.1.14), evaluated from 40 levels down (converged to double precision there). */
+double orc_erfcx(double y)
+{
+    if (y < 26.0) {
+        const double h = y * y;
+        const double l = fma(y, y, -h);
+        return exp(h) * (1.0 + l) * erfc(y);
+    }
+    double t = y;
+    for (int k = 40; k >= 1; --k) t = y + (0.5 * k) / t;
+    return 1.0 / (sqrt(acos(-1.0)) * t);
+}
+
+/* Sigma series of Zaghloul & Ali, Alg. 916, truncated as cross_kernal.cu:61-111 truncates it:
+ * n = 1..4, the third series centred on n0 = ceil(x/a) */
+static void orc_sigma916(double x, double y, double *s1, double *s2, double *s3, double a)
+{
+    double f, f3p, f3n, an, an3p, an3n;
+    const double yy = y * y;
+    const int n0 = (int)(ceil(x / a));
+    *s1 = 0.0; *s2 = 0.0; *s3 = 0.0;
+    for (int n = 1; n < 5; ++n) {
+        const int n3p = n0 + n - 1, n3n = n0 - n;
+        an = a * n;
+        an3p = a * n3p;
+        an3n = a * n3n;
+        f = 1.0 / (an * an + yy);
+        f3p = 1.0 / (an3p * an3p + yy);
+        f3n = 1.0 / (an3n * an3n + yy);
+        *s1 += f * exp(-(an * an + x * x));
+        *s2 += f * exp(-(an + x) * (an + x));
+        *s3 += f3p * exp(-(an3p - x) * (an3p - x));
+        if (n3n >= 1) *s3 += f3n * exp(-(an3n - x) * (an3n - x));
+    }
+}
+
+double orc_voigt916(double x, double y, double a)
+{
+    /* cross_kernal.cu:121-147 (equation 13 of Alg. 916) */
+    const double ex2 = exp(-x * x);
+    if (x == 0) return orc_erfcx(y);
+    if (y == 0) return ex2;
+    double s1, s2, s3;
+    orc_sigma916(x, y, &s1, &s2, &s3, a);
+    const double xy = x * y;
+    const double a2ipi = 2.0 * a / M_PI;
+    const double cos2xy = cos(2.0 * xy);
+    const double sinxy = sin(xy);
+    double t1 = ex2 * orc_erfcx(y) * cos2xy;
+    t1 += a2ipi * x * sinxy * ex2 * sinxy / xy;
+    t1 += a2ipi * y * (-cos2xy * s1 + 0.5 * (s2 + s3));
+    return t1;
+}
+
+double orc_gh3(double x, double y, double xxyy)
+{
+    /* cross_kernal.cu:183-186, constants and order as written (PI the defines.h literal) */
+    return 1.1181635900 * y / (ORC_PI * xxyy) + 2.0 * 0.2954089751 * (xxyy + 1.5) /
+           ((xxyy + 1.5) * (xxyy + 1.5) - 4 * x * x * 1.5);
+}
+
 double orc_gamma_t(const orc_consts *c, double gamma, double nexp)
 {
     return gamma * pow(c->ref_temp / c->temperature, nexp);   /* OpenMPManager.cpp:204 */
@@ -159,7 +235,7 @@ long orc_window(const orc_consts *c, double nu, double cutoff_i, const double *f
                 int points, int start_idx, int *ib, int *ie)
 {
     (void)npoints_total;
-    if (c->convention == ORC_CONV_GPU) {
+    if (c->convention == ORC_CONV_GPU || c->profile == ORC_GAUSSIAN) {
         /* cross_kernal.cu:954-957 / :686-687: bin b of the chunk takes the line iff -C <= nu - f_b <= C */
         int s = start_idx, e = start_idx + points;
         *ib = first_true_le(nu, cutoff_i, freq, s, e);
@@ -207,6 +283,55 @@ typedef struct slice_job {
     double *grid;    /* private grid indexed by absolute bin - start_idx */
 } slice_job;
 
+/* The CUDA manager's Gaussian (cross_kernal.cu:567-625) and no-gamma Voigt (:860-922) sums of
+ * one line, bin by bin in the line's per-bin window (|nu - f| <= C, inclusive both ends).
+ * Gaussian: ln2 is the defines.h:4 literal (the reference never sets cross_constants.ln2,
+ * GPUManager.cpp:24-47) and every line in the window is summed (the reference's sorted-order
+ * early exit tests f - nu > hw, which stops at the first line BELOW the window).  No-gamma
+ * Voigt: gamma_L = 0.05, window 500 gamma_L, Doppler width nu sqrt(2 ln2 k T / m) / c as
+ * written there (no Avogadro factor), Alg. 916 where x^2 + y^2 < 100 (at |x|), else the 3-point
+ * form.  With the mean mass in amu, as the reference's drivers pass it, the Doppler width is
+ * ~1e-16 cm^-1 and every bin takes the 3-point form. */
+static void plain_line(const slice_job *jb, size_t i)
+{
+    const orc_consts *c = jb->c;
+    double *grid = jb->grid - jb->start_idx;
+    const double nu = jb->nu[i];
+    const double abscoef = orc_line_intensity_plain(c, jb->elow[i], nu, jb->aif[i], jb->gns[i]);
+    if (abscoef == 0.0) return;
+    const double cut = c->cutoff;
+    const int s = jb->start_idx, e = jb->start_idx + jb->points;
+    const int ib = first_true_le(nu, cut, jb->freq, s, e);
+    int ie = first_true_lt(nu, -cut, jb->freq, s, e);
+    if (ie < ib) ie = ib;
+    if (c->profile == ORC_GAUSSIAN) {
+        const double x_ = sqrt(ORC_LN2) / c->half_width;          /* cross_kernal.cu:582 */
+        const double x0 = x_ * c->dfreq * 0.5;
+        for (int j = ib; j < ie; ++j) {
+            const double dfreq_ = jb->freq[j] - nu;                    /* :594 */
+            const double xp = x_ * (dfreq_) + x0;
+            const double xm = x_ * (dfreq_) - x0;
+            grid[j] += abscoef * (erf(xp) - erf(xm));
+        }
+        return;
+    }
+    const double gammaL = 0.05;
+    const double dpwcoeff = sqrt(2.0 * ORC_LN2 * ORC_BOLTZ * c->temperature / (c->mean_mass)) / ORC_VELLGT;  /* :872 */
+    for (int j = ib; j < ie; ++j) {
+        const double dfreq_ = nu - jb->freq[j];                        /* :884 */
+        const double gammaG = nu * dpwcoeff;
+        const double x = ORC_SQRTLN2 * dfreq_ / gammaG;
+        const double y = ORC_SQRTLN2 * gammaL / gammaG;
+        const double xxyy = x * x + y * y;
+        /* the series at |x|: the reference passes the signed offset with its x < 0 guard
+         * commented out (cross_kernal.cu:77), which adds n <= 0 terms for bins above the centre */
+        if (xxyy < 100.0)
+            grid[j] += abscoef * ORC_SQRTLN2PI / (gammaG) * y * orc_voigt916(fabs(x), y, 1.0);
+        else
+            grid[j] += abscoef * (ORC_SQRTLN2PI / gammaG) * orc_gh3(x, y, xxyy);
+    }
+}
+
 static void *run_slice(void *arg)
 {
     slice_job *jb = (slice_job *)arg;
@@ -216,6 +341,10 @@ static void *run_slice(void *arg)
     double *grid = jb->grid - jb->start_idx;
     for (size_t i = jb->begin; i < jb->end; ++i) {
         double nu_if = jb->nu[i];
+        if (c->profile == ORC_GAUSSIAN || (c->profile == ORC_VOIGT && !jb->gamma)) {
+            plain_line(jb, i);
+            continue;
+        }
         if (c->profile == ORC_VOIGT || c->convention == ORC_CONV_OPENMP)
             if (nu_if == 0) nu_if = 1e-6;
         const double abscoef = orc_line_intensity(c, jb->elow[i], jb->nu[i], jb->aif[i], jb->gns[i]);
@@ -252,7 +381,8 @@ int orc_cross_section(const orc_consts *c, size_t nlines, const double *elow, co
                       int nthreads, double *out)
 {
     if (!c || nthreads < 1 || points < 0 || start_idx < 0 || start_idx + points > npoints_total) return 1;
-    if (c->profile == ORC_VOIGT && (!gamma || !nexp)) return 1;
+    if (c->profile == ORC_VOIGT && (!gamma != !nexp)) return 1;      /* both or neither */
+    if (c->profile == ORC_GAUSSIAN && !(c->half_width > 0)) return 1;
     if (nlines > 0x7fffffff) return 1;
     const int nener = (int)nlines;
     /* OpenMPManager.cpp:186-188: slice size computed in float */
@@ -298,4 +428,14 @@ void orc_humlic_vec(size_t n, const double *x, const double *y, double *out)
 void orc_erf_vec(size_t n, const double *x, double *out)
 {
     for (size_t i = 0; i < n; ++i) out[i] = erf(x[i]);
+}
+
+void orc_voigt916_vec(size_t n, const double *x, const double *y, double a, double *out)
+{
+    for (size_t i = 0; i < n; ++i) out[i] = orc_voigt916(x[i], y[i], a);
+}
+
+void orc_erfcx_vec(size_t n, const double *x, double *out)
+{
+    for (size_t i = 0; i < n; ++i) out[i] = orc_erfcx(x[i]);
 }

@@ -21,6 +21,10 @@
  *   Doppler accumulation OpenMPManager.cpp:138-181
  *   thread reduction     OpenMPManager.cpp:118-132
  *   chunk range          BaseProfile.cpp:55-57
+ *   Gaussian profile     cross_kernal.cu:471-496 (intensity), :567-625 (accumulation),
+ *                        GPUManager.cpp:24-47 (constants; ln2 taken from defines.h:4)
+ *   Voigt without gamma  cross_kernal.cu:731-756 (intensity), :860-922 (accumulation),
+ *                        :61-147 (Zaghloul & Ali Alg. 916), :183-186 (3-point Gauss-Hermite)
  *
  * Parity pins (tests/test_oracle.py): every function here is checked against
  * the golden fixtures written by the reference's own OpenMPManager (built
@@ -46,8 +50,10 @@ extern "C" {
 #define ORC_BOLTZ 1.380658e-16
 #define ORC_NA 6.0221412927e23
 #define ORC_DOPPLER_CUTOFF 100.0
+#define ORC_SQRTLN2PI 0.46971863935
+#define ORC_SQRTLN2 0.83255461115
 
-enum { ORC_DOPPLER = 1, ORC_VOIGT = 2 };          /* Input.h:9-15 numbering */
+enum { ORC_GAUSSIAN = 0, ORC_DOPPLER = 1, ORC_VOIGT = 2 };   /* Input.h:9-15 numbering */
 enum { ORC_CONV_OPENMP = 0, ORC_CONV_GPU = 1 };
 
 typedef struct orc_consts {
@@ -61,6 +67,7 @@ typedef struct orc_consts {
     double ref_temp;        /* K (500 for ExoMol decks, VoigtProfile.cpp:14) */
     double cutoff;          /* window half-width, cm^-1 */
     double doppler_widths;  /* >0: Doppler per-line cutoff = k*alpha_i + dfreq */
+    double half_width;      /* Gaussian HWHM, cm^-1 (InitializeConstants' half_width) */
 } orc_consts;
 
 /* Humlicek W4 Re w(x+iy), profiles.cpp:9-42 */
@@ -112,7 +119,21 @@ void orc_chunk_range(double min_nu, double max_nu, double hw, double start_nu, d
 long long orc_count_evals(const orc_consts *c, size_t nlines, const double *nu, const double *freq,
                           int npoints_total, int points, int start_idx);
 
+/* erfcx(y) = exp(y^2) erfc(y), y >= 0: exp of the exactly split square times libm erfc
+ * below 26, Laplace's continued fraction (Abramowitz & Stegun 7.1.14) above. */
+double orc_erfcx(double y);
+/* Zaghloul & Ali (2011, ACM TOMS 38, Alg. 916) Voigt V(x,y) as cross_kernal.cu:61-147 states
+ * it: the sigma series truncated to n = 1..4 around n0 = ceil(x/a). */
+double orc_voigt916(double x, double y, double a);
+/* the 3-point Gauss-Hermite expression of cross_kernal.cu:183-186, as written */
+double orc_gh3(double x, double y, double xxyy);
+/* S of the CUDA manager's Gaussian and no-gamma Voigt steps (cross_kernal.cu:471-496, 731-756):
+ * cmcoef with acos(-1), no ISQRTPI, nu == 0 gives 0 */
+double orc_line_intensity_plain(const orc_consts *c, double elow, double nu, double aif, int gns);
+
 /* element-wise orc_humlic / libm erf over arrays (test convenience) */
+void orc_voigt916_vec(size_t n, const double *x, const double *y, double a, double *out);
+void orc_erfcx_vec(size_t n, const double *x, double *out);
 void orc_humlic_vec(size_t n, const double *x, const double *y, double *out);
 void orc_erf_vec(size_t n, const double *x, double *out);
 

@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-DOPPLER, VOIGT = 1, 2
+GAUSSIAN, DOPPLER, VOIGT = 0, 1, 2
 CONV_OPENMP, CONV_GPU = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -31,7 +31,7 @@ class OrcConsts(ctypes.Structure):
     _fields_ = [("profile", ctypes.c_int), ("convention", ctypes.c_int), ("temperature", ctypes.c_double),
                 ("partition", ctypes.c_double), ("dfreq", ctypes.c_double), ("mean_mass", ctypes.c_double),
                 ("pressure", ctypes.c_double), ("ref_temp", ctypes.c_double), ("cutoff", ctypes.c_double),
-                ("doppler_widths", ctypes.c_double)]
+                ("doppler_widths", ctypes.c_double), ("half_width", ctypes.c_double)]
 
 
 def build() -> str:
@@ -53,6 +53,13 @@ def lib():
         L.orc_humlic.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_humlic_vec.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
         L.orc_erf_vec.argtypes = [ctypes.c_size_t, _dp, _dp]
+        L.orc_erfcx_vec.argtypes = [ctypes.c_size_t, _dp, _dp]
+        L.orc_voigt916_vec.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_gh3.restype = ctypes.c_double
+        L.orc_gh3.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_line_intensity_plain.restype = ctypes.c_double
+        L.orc_line_intensity_plain.argtypes = [ctypes.POINTER(OrcConsts), ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int]
         L.orc_partition.restype = ctypes.c_double
         L.orc_partition.argtypes = [ctypes.c_size_t, _dp, _ip, ctypes.c_double]
         for name in ("orc_beta",):
@@ -100,11 +107,33 @@ def _i(a: np.ndarray):
 
 def consts(profile: int, temperature: float, partition: float, dfreq: float, mean_mass: float,
            pressure: float = 1.0, ref_temp: float = 500.0, cutoff: Optional[float] = None,
-           convention: int = CONV_OPENMP, doppler_widths: float = 0.0) -> OrcConsts:
+           convention: int = CONV_OPENMP, doppler_widths: float = 0.0, half_width: float = 0.0) -> OrcConsts:
+    """Constants of one point.  Default cutoff: the reference rule of the profile (Gaussian:
+    10 half-widths, cross_kernal.cu:578).  A Voigt line list without gamma takes the CUDA
+    manager's fixed window NOGAMMA_CUTOFF (pass it as cutoff)."""
     if cutoff is None:
-        cutoff = lib().orc_reference_cutoff(profile, pressure)
+        cutoff = 10.0 * half_width if profile == GAUSSIAN else lib().orc_reference_cutoff(profile, pressure)
     return OrcConsts(profile, convention, temperature, partition, dfreq, mean_mass, pressure, ref_temp, cutoff,
-                     doppler_widths)
+                     doppler_widths, half_width)
+
+
+# window of the Voigt sum without per-line gamma: 500 gamma_L, gamma_L = 0.05 (cross_kernal.cu:870-889)
+NOGAMMA_CUTOFF = 500.0 * 0.05
+
+
+def voigt916(x, y, a: float = 1.0) -> np.ndarray:
+    x = np.ascontiguousarray(np.broadcast_to(np.asarray(x, np.float64), np.broadcast(x, y).shape)).ravel()
+    y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float64), x.shape)).ravel()
+    out = np.empty_like(x)
+    lib().orc_voigt916_vec(x.size, _d(x), _d(y), a, _d(out))
+    return out
+
+
+def erfcx(x) -> np.ndarray:
+    x = np.ascontiguousarray(np.asarray(x, np.float64)).ravel()
+    out = np.empty_like(x)
+    lib().orc_erfcx_vec(x.size, _d(x), _d(out))
+    return out
 
 
 def humlic(x, y) -> np.ndarray:

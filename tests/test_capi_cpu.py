@@ -26,8 +26,8 @@ def test_status_codes_without_device_or_with_bad_args():
     L = capi.load()
     assert L.gx_version().startswith(b"gx")
     h = ctypes.c_void_p()
-    # unknown profile is rejected before any device work
-    assert L.gx_create(0, 0, 0, ctypes.byref(h)) == capi.GX_INVALID_ARG
+    # an unserved profile (PSUEDO_VOIGT, Input.h:13) is rejected before any device work
+    assert L.gx_create(0, 3, 0, ctypes.byref(h)) == capi.GX_INVALID_ARG
     assert b"profile" in L.gx_last_error()
     assert L.gx_create(0, capi.VOIGT, 0, None) == capi.GX_INVALID_ARG
     # NULL contexts

@@ -152,6 +152,67 @@ def test_doppler_gpu_convention_vs_oracle(T, widths):
     assert t.evals == po.count_evals(c, L.nu[nz], freq, freq.size, 0)
 
 
+@pytest.mark.parametrize("T,hw,cutoff", [(296.0, 0.05, 0.0), (1500.0, 0.2, 0.0), (800.0, 0.01, 0.0),
+                                         (800.0, 0.3, 1.5)])
+def test_gaussian_vs_oracle(T, hw, cutoff):
+    """The fixed-half-width Gaussian (cross_kernal.cu:471-625): window 10 HWHM (or a given
+    cutoff), erf-binned, FP64, against the oracle at the parity bar; windows counted as the
+    oracle's per-bin rule counts them."""
+    ll = make_lines(1, 20000, 15000.0, 200_000, 10000.0, 30.01, voigt=False)
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 1990.0) & (ll.nu < 2110.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(2000.0, 2100.0, 20000)
+    print(f"gaussian T={T} hw={hw} cutoff={cutoff} lines={L.n}", flush=True)
+    with Mi355xManager(capi.GAUSSIAN, cutoff=cutoff) as m:
+        m.InitializeVectors(freq.size)
+        m.TransferFreq(freq, np.zeros_like(freq), freq.size)
+        m.InitializeConstants(hw, T, Q, d, 30.01, 1.0, 296.0)
+        m.TransferVectors(L.n, L.elow, L.nu, L.aif, L.gns)
+        m.ExecuteCrossSection(freq.size, L.n, 0)
+        out = np.zeros_like(freq)
+        m.TransferResults(None, out, freq.size)
+        t = m.last_timing()
+    c = po.consts(po.GAUSSIAN, T, Q, d, 30.01, 1.0, 296.0, half_width=hw, cutoff=cutoff if cutoff > 0 else None)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, None, None, freq, freq.size, 0, 4)
+    assert rel_err(out, ref) <= FP64_TOL
+    assert t.evals == po.count_evals(c, L.nu, freq, freq.size, 0)
+
+
+@pytest.mark.parametrize("mass", [17.03, 17.03 / 6.0221412927e23], ids=["amu", "grams"])
+@pytest.mark.parametrize("T", [300.0, 2000.0])
+def test_voigt_without_gamma_vs_oracle(voigt_lines, T, mass):
+    """A Voigt chunk uploaded without gamma and n takes the CUDA manager's sum for it
+    (cross_kernal.cu:860-922): gamma_L = 0.05, window 25 cm^-1, Alg. 916 near the centre and the
+    3-point form beyond.  The mean mass in amu (the reference drivers' unit: its Doppler width
+    there lacks Avogadro's number, so every bin takes the 3-point form) and in grams (physical
+    width: Alg. 916 within ~10 Doppler widths).  FP64 against the oracle, mixed constants too."""
+    ll = voigt_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    sel = np.nonzero((ll.nu > 2950.0) & (ll.nu < 3150.0))[0]
+    L = ll.subset(sel)
+    freq, d = make_grid(3000.0, 3100.0, 20000)
+    print(f"voigt without gamma T={T} mass={mass:.3e} lines={L.n}", flush=True)
+    outs = []
+    for prec in (capi.PREC_FP64, capi.PREC_MIXED):    # no mixed form for this sum: FP64 either way
+        with Mi355xManager(capi.VOIGT, precision=prec) as m:
+            m.InitializeVectors(freq.size)
+            m.TransferFreq(freq, np.zeros_like(freq), freq.size)
+            m.InitializeConstants(0.0, T, Q, d, mass, 1.0, 500.0)
+            m.TransferVectors(L.n, L.elow, L.nu, L.aif, L.gns)
+            m.ExecuteCrossSection(freq.size, L.n, 0)
+            out = np.zeros_like(freq)
+            m.TransferResults(None, out, freq.size)
+            t = m.last_timing()
+        outs.append(out)
+    assert np.array_equal(outs[0], outs[1])
+    c = po.consts(po.VOIGT, T, Q, d, mass, 1.0, 500.0, cutoff=po.NOGAMMA_CUTOFF)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, None, None, freq, freq.size, 0, 4)
+    assert rel_err(outs[0], ref) <= FP64_TOL
+    cw = po.consts(po.VOIGT, T, Q, d, mass, 1.0, 500.0, cutoff=po.NOGAMMA_CUTOFF, convention=po.CONV_GPU)
+    assert t.evals == po.count_evals(cw, L.nu, freq, freq.size, 0)
+
+
 @pytest.mark.parametrize("T,P", [(1500.0, 0.1), (300.0, 1e-3), (3000.0, 10.0), (1000.0, 1.0)])
 def test_voigt_mixed_vs_fp64_oracle(voigt_lines, T, P):
     ll = voigt_lines

@@ -250,3 +250,104 @@ def test_linearity_superposition_threads_translation():
     assert d2 == d
     shifted = run(grid=shifted_grid)
     np.testing.assert_allclose(shifted[:8000], full[40:8040], rtol=1e-15)
+
+
+# ------------------------------------------ Gaussian and no-gamma Voigt (CUDA-manager sums)
+def _plain_line(profile, nu, S_target, freq, d, T=1000.0, mass=17.03, **kw):
+    """Grid of one line of intensity S_target (A chosen so that the plain intensity equals it)."""
+    c = po.consts(profile, T, 500.0, d, mass, 1.0, 500.0, **kw)
+    s1 = po.lib().orc_line_intensity_plain(c, 1000.0, nu, 1.0, 1)
+    aif = np.array([S_target / s1])
+    out = po.cross_section(c, np.array([1000.0]), np.array([nu]), aif, np.array([1], np.int32), None, None, freq,
+                           freq.size, 0, 1)
+    return out, c
+
+
+def test_plain_intensity_closed_form():
+    T, Q = 1500.0, 3210.0
+    c = po.consts(po.GAUSSIAN, T, Q, 0.01, 17.03, 1.0, 500.0, half_width=0.1)
+    beta = 6.6260693e-27 * 2.99792458e10 / (1.380658e-16 * T)
+    E, nu, A, g = 2345.6, 1234.5, 3.2, 7
+    want = A * g * math.exp(-beta * E) * (-math.expm1(-beta * nu)) / (8 * math.pi * 2.99792458e10 * nu * nu * Q)
+    assert po.lib().orc_line_intensity_plain(c, E, nu, A, g) == pytest.approx(want, rel=1e-13)
+    assert po.lib().orc_line_intensity_plain(c, E, 0.0, A, g) == 0.0      # nu = 0 adds nothing
+
+
+def test_erfcx_against_scipy():
+    from scipy import special
+    y = np.concatenate([np.linspace(0.0, 60.0, 6001), 10 ** np.linspace(-8, 3, 400)])
+    got, want = po.erfcx(y), special.erfcx(y)
+    assert np.max(np.abs(got - want) / want) <= 2e-15
+
+
+def test_voigt916_against_scipy_wofz():
+    """Alg. 916 truncated to four terms (a = 1) against the Faddeeva function: within the
+    3e-4 the truncation leaves, exact at x = 0 (erfcx) and y = 0 (Gaussian)."""
+    from scipy import special
+    X, Y = np.meshgrid(np.linspace(0.001, 9.9, 240), 10 ** np.linspace(-3, 1, 160))
+    m = X ** 2 + Y ** 2 < 100
+    v = po.voigt916(X[m], Y[m])
+    w = special.wofz(X[m] + 1j * Y[m]).real
+    rel = np.abs(v - w) / w
+    assert rel.max() <= 3e-4 and np.median(rel) <= 1e-4
+    y = np.linspace(0.01, 9.0, 50)
+    assert np.allclose(po.voigt916(np.zeros_like(y), y), special.wofz(1j * y).real, rtol=1e-14, atol=0)
+    x = np.linspace(0.01, 9.0, 50)
+    assert np.allclose(po.voigt916(x, np.zeros_like(x)), np.exp(-x * x), rtol=1e-15, atol=0)
+
+
+def test_gaussian_telescoping_sum_and_half_width():
+    """The binned Gaussian telescopes to 2S over a window of +-10 HWHM, and its value falls to
+    one half one HWHM from the centre (erf difference of a narrow bin ~ bin x Gaussian)."""
+    hw = 0.05
+    freq, d = po.make_grid(990.0, 1010.0, 200_000)             # 1e-4 cm^-1 bins: HW / 500
+    nu = 1000.0 + 0.37 * d
+    out, c = _plain_line(po.GAUSSIAN, nu, 2.5e-20, freq, d, half_width=hw)
+    assert out.sum() == pytest.approx(2 * 2.5e-20, rel=1e-12)
+    prof = lambda f: np.interp(f, freq, out)
+    assert prof(nu + hw) / prof(nu) == pytest.approx(0.5, rel=1e-5)   # O((bin/HW)^2) from binning
+    assert prof(nu - hw) / prof(nu) == pytest.approx(0.5, rel=1e-5)
+    # per-bin window |nu - f| <= 10 HW, inclusive at both ends
+    nz = np.nonzero(out)[0]
+    inside = np.nonzero(np.abs(nu - freq) <= 10 * hw)[0]
+    assert nz.min() >= inside.min() and nz.max() <= inside.max()
+    assert po.count_evals(c, np.array([nu]), freq, freq.size, 0) == inside.size
+
+
+def test_nogamma_voigt_lorentz_limit_with_mass_in_amu():
+    """With the mean mass in amu the no-gamma sum's Doppler width (no Avogadro factor) is
+    ~1e-16 cm^-1, every bin takes the 3-point form, and the profile reduces (to O(gG/gamma_L)) to
+    S 1.1181635900 (SQRTLN2PI/SQRTLN2)/PI gamma_L/(dnu^2 + gamma_L^2), gamma_L = 0.05."""
+    freq, d = po.make_grid(3000.0, 3100.0, 20_000)
+    nu = 3050.0 + 0.3 * d
+    S = 1.7e-21
+    out, c = _plain_line(po.VOIGT, nu, S, freq, d, cutoff=po.NOGAMMA_CUTOFF)
+    g = 0.05
+    dnu = nu - freq
+    want = S * 1.1181635900 * (0.46971863935 / 0.83255461115) / 3.14159265359 * g / (dnu ** 2 + g ** 2)
+    win = np.abs(dnu) <= 25.0
+    np.testing.assert_allclose(out[win], want[win], rtol=1e-11, atol=0)   # x, y ~ 1e16: squares ~ 1e32
+    assert not out[~win].any()
+
+
+def test_nogamma_voigt_alg916_centre_with_mass_in_grams():
+    """With the molecular mass in grams the Doppler width is physical: near the centre
+    (x^2 + y^2 < 100) the sum is S SQRTLN2PI/gG y V916(|x|, y), within 3e-4 of
+    S SQRTLN2PI/gG y Re w(x + iy), and symmetric about the centre."""
+    from scipy import special
+    freq, d = po.make_grid(2999.0, 3001.0, 4000)
+    nu = 3000.0
+    S = 1e-20
+    mass_g = 17.03 / 6.0221412927e23
+    out, c = _plain_line(po.VOIGT, nu, S, freq, d, T=1000.0, mass=mass_g, cutoff=po.NOGAMMA_CUTOFF)
+    gG = nu * math.sqrt(2.0 * 0.69314718056 * 1.380658e-16 * 1000.0 / mass_g) / 2.99792458e10
+    y = 0.83255461115 * 0.05 / gG
+    x = 0.83255461115 * (nu - freq) / gG
+    core = x ** 2 + y ** 2 < 100
+    assert core.sum() > 100 and (~core).sum() > 100
+    w = special.wofz(np.abs(x[core]) + 1j * y).real
+    want = S * 0.46971863935 / gG * y * w
+    assert np.max(np.abs(out[core] - want) / want) <= 3e-4
+    mid = 2000                                                      # freq[2000] == nu
+    assert freq[mid] == nu
+    np.testing.assert_allclose(out[mid - 500:mid][::-1], out[mid + 1:mid + 501], rtol=1e-12)
