@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgx.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gx.h")
 HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", h)
-                      for h in ("gx_ingest.h", "gx_output.h", "gx_hitran.h")]
+                      for h in ("gx_ingest.h", "gx_output.h", "gx_hitran.h", "gx_tips.h")]
 
 GX_OK, GX_INVALID_ARG, GX_OUT_OF_MEMORY, GX_NO_DEVICE, GX_DEVICE_ERROR, GX_BUSY, GX_NOT_FOUND = range(7)
 GAUSSIAN, DOPPLER, VOIGT = 0, 1, 2
@@ -192,6 +192,12 @@ def load():
                                                _dp])
     _sig(L, "gx_opacity_read_info", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(OpacityInfo)])
     _sig(L, "gx_opacity_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Point), sz, _dp, sz])
+    _sig(L, "gx_tips_load", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(P)])
+    _sig(L, "gx_tips_load_text", ctypes.c_int, [P, sz, ctypes.POINTER(P)])
+    _sig(L, "gx_tips_free", None, [P])
+    _sig(L, "gx_tips_q", ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp])
+    _sig(L, "gx_tips_isotopes", ctypes.c_int, [P, ctypes.c_int])
+    _sig(L, "gx_tips_lagrange", ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_double, _dp])
     _sig(L, "gx_last_error", ctypes.c_char_p, [])
     _sig(L, "gx_version", ctypes.c_char_p, [])
     _lib = L

@@ -240,12 +240,20 @@ def format_trans(id_f: np.ndarray, id_i: np.ndarray, aif: np.ndarray) -> np.ndar
 
 
 def read_hitran(src: Union[str, bytes, os.PathLike, np.ndarray], pressure: float = 1.0, mixture: float = 1.0,
-                ref_partition: float = 0.0, lo: float = -np.inf, hi: float = np.inf, nthreads: int = 0):
+                ref_partition: float = 0.0, lo: float = -np.inf, hi: float = np.inf, nthreads: int = 0,
+                tips=None):
     """HITRAN .par text read as HITRANStateReader and the Voigt driver read it
     (HITRANStateReader.cpp:3-250, VoigtProfile.cpp:61-75): (Lines, molecule id, isotope).
-    ref_partition = Q(296 K), used for records whose A field is 0."""
+    ref_partition = Q(296 K), used for records whose A field is 0; when it is not given and
+    ``tips`` (a gpu_cross_amd.tips.Tips) is, it is TIPS-2011 Q(296 K) of the first record's
+    molecule and isotopologue, as HITRANStateReader::OpenFile takes it (HITRANStateReader.cpp:81-95)."""
     t = _text(src if isinstance(src, np.ndarray) else _read_bytes(src))
     L = capi.load()
+    if ref_partition <= 0 and tips is not None and t.size:
+        nl = np.nonzero(t == 10)[0]
+        first = np.ascontiguousarray(t[: (nl[0] + 1) if nl.size else t.size])
+        _, mol0, iso0 = read_hitran(first, pressure, mixture, 1.0)
+        ref_partition = tips.q(mol0, iso0, 296.0)[1]
     o = capi.HitranOpts(float(pressure), float(mixture), float(ref_partition), float(lo), float(hi), int(nthreads))
     n, rec = ctypes.c_size_t(0), ctypes.c_size_t(0)
     mol, iso = ctypes.c_int(0), ctypes.c_int(0)
