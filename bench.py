@@ -63,7 +63,11 @@ def cpu_baseline(ll, wl, Q, _gam=None, target_s=15.0):
         cores = len(os.sched_getaffinity(0))
     except Exception:
         pass
-    cores = min(cores, 16)
+    affinity = cores
+    # the host's CPU share for this GPU: OMP_NUM_THREADS when the environment sets it, else every
+    # core this process may run on
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = max(1, min(cores, int(os.environ["OMP_NUM_THREADS"])))
     P = 1.0
     gam = ll.gamma_at(P)
     c = po.consts(po.VOIGT, wl["T"], Q, d, ll.mass, P, 500.0, cutoff=wl["cutoff"])
@@ -85,8 +89,8 @@ def cpu_baseline(ll, wl, Q, _gam=None, target_s=15.0):
     port_rate = n_ev / dt
     sample = (f"bins [{j0},{j0 + bins}) of the same grid ({bins} bins) with every line touching them "
               f"({b - a} lines, {n_ev:.3e} evals), P = 1 atm")
-    res = {"value": port_rate, "unit": "evals/s", "cores": cores, "kind": "port", "sample": sample,
-           "port_seconds": dt}
+    res = {"value": port_rate, "unit": "line-bin pairs/s", "cores": cores, "kind": "port", "sample": sample,
+           "port_seconds": dt, "host_cpu": cpu_model(), "affinity_cores": affinity}
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if os.path.exists(harness) and bins * 8 >= 64 * 1024:
         with tempfile.TemporaryDirectory() as tmp:
@@ -107,6 +111,17 @@ def cpu_baseline(ll, wl, Q, _gam=None, target_s=15.0):
                            sample=sample + "; the reference's own OpenMPManager (oracle/_ref, built from its "
                                            "sources), timed around its BaseManager calls")
     return res
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def calibrate_copy(mib: int) -> None:
@@ -167,8 +182,14 @@ def bench_table(args, wl):
     value = evals * args.steps / wall
     ms_step = wall / args.steps * 1e3
     res = {
-        "metric": "Voigt line-bin evals/s (48-point (T,P) table, mixed FP32+Kahan, FP64 intensity; 1e-5 rel err)",
-        "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "metric": "Voigt line-bin evals/s + ms per (T,P) xsec at 1e-5 rel err; serving p50/p99",
+        "value": value, "unit": "line-bin pairs/s",
+        "value_definition": "line-bin pairs (line i, bin j) with ib_i <= j < ie_i from the FP64 windows, "
+                            "summed over the 48 grids, per second of table wall time; far_field.share of them "
+                            "are summed from multipole moments, the rest evaluated one by one "
+                            "(direct_evals_per_s)",
+        "direct_evals_per_s": (r1_evals + core_evals) * args.steps / wall,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"table_48: {ll.n} lines, 8 T log-spaced 300-3000 K x 6 P log-spaced 1e-4-100 atm, "
@@ -192,6 +213,26 @@ def bench_table(args, wl):
     mixed_grids = [tab.read(p) for p in range(len(pts))]
     tab.close()
     if not args.no_fp64:
+        # the same mixed table with the far field off: every pair evaluated one by one (256-bin
+        # blocks, the launch shape that is fastest without the far field)
+        os.environ["GX_FAR"], os.environ["GX_MIXED_R"] = "0", "4"
+        try:
+            tnf = Table(lines, start, end, M, pts, ll.mass, 500.0, wl["cutoff"], use_graph=True, autotune=False)
+            tnf.run()
+            tnf.wait()
+            ws = []
+            for _ in range(2):
+                t1 = time.perf_counter()
+                tnf.run()
+                tnf.wait()
+                ws.append(time.perf_counter() - t1)
+            tnf.close()
+        finally:
+            del os.environ["GX_FAR"], os.environ["GX_MIXED_R"]
+        nofar_ms = min(ws) * 1e3
+        log(f"mixed table without the far field: {nofar_ms:.1f} ms")
+        res["no_far_field"] = {"ms_per_table": nofar_ms, "pairs_per_s": evals / (nofar_ms * 1e-3),
+                               "form": "GX_FAR=0, 256-bin blocks: every pair evaluated (region I + core pass)"}
         f64 = Table(lines, start, end, M, pts, ll.mass, 500.0, wl["cutoff"], precision=capi.PREC_FP64,
                     use_graph=False, autotune=False)
         t1 = time.perf_counter()
@@ -212,6 +253,7 @@ def bench_table(args, wl):
                             "frac_fp64_peak": FLOP_PER_EVAL * tf.evals / fwall / PEAK_FP64,
                             "form": "48 separate per-point launches (intensity, ranges, FP64 accumulation)"}
         res["speedup_vs_fp64"] = value / res["fp64_path"]["evals_per_s"]
+        res["speedup_vs_fp64_direct"] = fwall * 1e3 / nofar_ms     # both evaluate every pair
         res["precision"] = {"max_rel_err": worst, "rms_rel_err": (rms_acc / max(nb, 1)) ** 0.5, "bins_checked": nb,
                             "against": "FP64 GPU path (oracle-parity 1e-10), all 48 grids"}
     lines.close()
@@ -589,7 +631,7 @@ def main():
     value = evals * args.steps / mixed["wall"]
     res = {
         "metric": "Voigt line-bin evals/s (mixed FP32+Kahan, FP64 intensity; 1e-5 rel err target)",
-        "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": value, "unit": "line-bin pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {ll.n} lines, {M} bins {start}-{end} cm^-1, T={T} K, P={P} atm, "

@@ -12,7 +12,8 @@ import re
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgx.so")
+# GX_LIB selects another build of the same library (the sanitizer build, libgx_asan.so)
+LIB_PATH = os.environ.get("GX_LIB") or os.path.join(HERE, "libgx.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gx.h")
 HEADERS = [HEADER] + [os.path.join(os.path.dirname(HERE), "include", h)
                       for h in ("gx_ingest.h", "gx_output.h", "gx_hitran.h", "gx_tips.h")]

@@ -126,6 +126,40 @@ void LineStore::release() {
   gns = nullptr;
   cap = 0;
   n = 0;
+  release_staging();
+}
+
+void LineStore::release_staging() {
+  hipFree(s_nu); hipFree(s_e); hipFree(s_a); hipFree(s_g); hipFree(s_gam); hipFree(s_n);
+  hipFree(s_idx_in); hipFree(s_idx_out); hipFree(s_tmp);
+  s_nu = s_e = s_a = s_gam = s_n = nullptr;
+  s_g = nullptr;
+  s_idx_in = s_idx_out = nullptr;
+  s_tmp = nullptr;
+  s_cap = 0;
+  s_tmp_cap = 0;
+  s_voigt = false;
+}
+
+int LineStore::reserve_staging(size_t nl, bool voigt) {
+  if (nl <= s_cap && (!voigt || s_voigt)) return GX_OK;
+  hipFree(s_nu); hipFree(s_e); hipFree(s_a); hipFree(s_g); hipFree(s_gam); hipFree(s_n);
+  hipFree(s_idx_in); hipFree(s_idx_out);
+  s_nu = s_e = s_a = s_gam = s_n = nullptr;
+  s_g = nullptr;
+  s_idx_in = s_idx_out = nullptr;
+  s_cap = 0;
+  s_voigt = false;
+  const size_t c = std::max(nl, s_cap);
+  hipError_t e;
+  if ((e = hipMalloc(&s_nu, c * 8)) || (e = hipMalloc(&s_e, c * 8)) || (e = hipMalloc(&s_a, c * 8)) ||
+      (e = hipMalloc(&s_g, c * 4)) || (e = hipMalloc(&s_idx_in, c * 4)) || (e = hipMalloc(&s_idx_out, c * 4)))
+    return hip_fail(e, "transfer_vectors: staging alloc");
+  if (voigt && ((e = hipMalloc(&s_gam, c * 8)) || (e = hipMalloc(&s_n, c * 8))))
+    return hip_fail(e, "transfer_vectors: staging alloc");
+  s_cap = c;
+  s_voigt = voigt;
+  return GX_OK;
 }
 
 // Upload n lines (host SoA) and leave them sorted by nu on the device.
@@ -158,40 +192,36 @@ int LineStore::upload(size_t nlines, const double* h_e, const double* h_nu, cons
     if ((e = hipStreamSynchronize(s))) return hip_fail(e, "transfer_vectors: sync");
     return GX_OK;
   }
-  // Unsorted chunk: stage raw arrays, radix-sort (nu, index) pairs, gather the rest.
+  // Unsorted chunk: stage raw arrays, radix-sort (nu, index) pairs, gather the rest.  The
+  // staging buffers are kept between uploads (grown only when a chunk is larger).
   const int ni = (int)nlines;
-  double *t_nu = nullptr, *t_e = nullptr, *t_a = nullptr, *t_gam = nullptr, *t_n = nullptr;
-  int* t_g = nullptr;
-  uint32_t *idx_in = nullptr, *idx_out = nullptr;
-  void* tmp = nullptr;
+  if ((rc = reserve_staging(nlines, voigt))) return rc;
+  if ((e = hipMemcpyAsync(s_nu, h_nu, nlines * 8, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(s_e, h_e, nlines * 8, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(s_a, h_a, nlines * 8, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(s_g, h_g, nlines * 4, hipMemcpyHostToDevice, s)))
+    return hip_fail(e, "transfer_vectors: copy");
+  if (voigt && ((e = hipMemcpyAsync(s_gam, h_gam, nlines * 8, hipMemcpyHostToDevice, s)) ||
+                (e = hipMemcpyAsync(s_n, h_n, nlines * 8, hipMemcpyHostToDevice, s))))
+    return hip_fail(e, "transfer_vectors: copy");
+  if ((e = launch_iota(s_idx_in, ni, s))) return hip_fail(e, "transfer_vectors: iota");
   size_t tmp_bytes = 0;
-  auto cleanup = [&]() {
-    hipFree(t_nu); hipFree(t_e); hipFree(t_a); hipFree(t_g); hipFree(t_gam); hipFree(t_n);
-    hipFree(idx_in); hipFree(idx_out); hipFree(tmp);
-  };
-  auto bail = [&](hipError_t err, const char* what) { cleanup(); return hip_fail(err, what); };
-  if ((e = hipMalloc(&t_nu, nlines * 8)) || (e = hipMalloc(&t_e, nlines * 8)) || (e = hipMalloc(&t_a, nlines * 8)) ||
-      (e = hipMalloc(&t_g, nlines * 4)) || (e = hipMalloc(&idx_in, nlines * 4)) || (e = hipMalloc(&idx_out, nlines * 4)))
-    return bail(e, "transfer_vectors: staging alloc");
-  if (voigt && ((e = hipMalloc(&t_gam, nlines * 8)) || (e = hipMalloc(&t_n, nlines * 8))))
-    return bail(e, "transfer_vectors: staging alloc");
-  if ((e = hipMemcpyAsync(t_nu, h_nu, nlines * 8, hipMemcpyHostToDevice, s)) ||
-      (e = hipMemcpyAsync(t_e, h_e, nlines * 8, hipMemcpyHostToDevice, s)) ||
-      (e = hipMemcpyAsync(t_a, h_a, nlines * 8, hipMemcpyHostToDevice, s)) ||
-      (e = hipMemcpyAsync(t_g, h_g, nlines * 4, hipMemcpyHostToDevice, s)))
-    return bail(e, "transfer_vectors: copy");
-  if (voigt && ((e = hipMemcpyAsync(t_gam, h_gam, nlines * 8, hipMemcpyHostToDevice, s)) ||
-                (e = hipMemcpyAsync(t_n, h_n, nlines * 8, hipMemcpyHostToDevice, s))))
-    return bail(e, "transfer_vectors: copy");
-  if ((e = launch_iota(idx_in, ni, s))) return bail(e, "transfer_vectors: iota");
-  if ((e = sort_pairs_by_nu(nullptr, tmp_bytes, t_nu, nu, idx_in, idx_out, ni, s)))
-    return bail(e, "transfer_vectors: sort size");
-  if ((e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)))) return bail(e, "transfer_vectors: sort alloc");
-  if ((e = sort_pairs_by_nu(tmp, tmp_bytes, t_nu, nu, idx_in, idx_out, ni, s))) return bail(e, "transfer_vectors: sort");
-  if ((e = launch_gather_lines(idx_out, t_e, t_a, t_g, t_gam, t_n, elow, aif, gns, gam, nexp, ni, s)))
-    return bail(e, "transfer_vectors: gather");
-  if ((e = hipStreamSynchronize(s))) return bail(e, "transfer_vectors: sync");
-  cleanup();
+  if ((e = sort_pairs_by_nu(nullptr, tmp_bytes, s_nu, nu, s_idx_in, s_idx_out, ni, s)))
+    return hip_fail(e, "transfer_vectors: sort size");
+  if (tmp_bytes > s_tmp_cap) {
+    if ((e = hipStreamSynchronize(s))) return hip_fail(e, "transfer_vectors: sync");
+    hipFree(s_tmp);
+    s_tmp = nullptr;
+    s_tmp_cap = 0;
+    if ((e = hipMalloc(&s_tmp, std::max<size_t>(tmp_bytes, 16)))) return hip_fail(e, "transfer_vectors: sort alloc");
+    s_tmp_cap = std::max<size_t>(tmp_bytes, 16);
+  }
+  if ((e = sort_pairs_by_nu(s_tmp, tmp_bytes, s_nu, nu, s_idx_in, s_idx_out, ni, s)))
+    return hip_fail(e, "transfer_vectors: sort");
+  if ((e = launch_gather_lines(s_idx_out, s_e, s_a, s_g, voigt ? s_gam : nullptr, voigt ? s_n : nullptr, elow, aif,
+                               gns, gam, nexp, ni, s)))
+    return hip_fail(e, "transfer_vectors: gather");
+  if ((e = hipStreamSynchronize(s))) return hip_fail(e, "transfer_vectors: sync");
   return GX_OK;
 }
 
@@ -662,7 +692,17 @@ struct gx_ctx {
 namespace {
 
 constexpr size_t kBytesPerLine = 44 + 60 + 52;   // resident + derived (FP64 + mixed) + sort staging
-constexpr size_t kMaxNtrans = size_t(1) << 30;   // keeps Nener within the int of ExecuteCrossSection
+// Chunk capacity reported by GetNtrans: the reference's drivers allocate six host arrays of
+// that many lines (VoigtProfile.cpp:28-33), so 2^26 lines (3 GB of host buffers) by default;
+// GX_MAX_NTRANS raises or lowers it (at most 2^30, which keeps N_ener an int).
+size_t max_ntrans() {
+  size_t v = size_t(1) << 26;
+  if (const char* env = std::getenv("GX_MAX_NTRANS")) {
+    const long long x = std::atoll(env);
+    if (x > 0) v = std::min<size_t>((size_t)x, size_t(1) << 30);
+  }
+  return v;
+}
 
 int set_device(gx_ctx* c) {
   hipError_t e = hipSetDevice(c->device);
@@ -727,7 +767,7 @@ int gx_initialize_vectors(gx_ctx* c, int npoints, size_t* ntrans_out) {
   c->M = npoints;
   const size_t grid_bytes = 16 * (size_t)npoints;
   const size_t avail = c->budget > grid_bytes ? c->budget - grid_bytes : 0;
-  c->ntrans = std::min(avail / kBytesPerLine, kMaxNtrans);   // GPUManager.cpp:55-58
+  c->ntrans = std::min(avail / kBytesPerLine, max_ntrans());   // GPUManager.cpp:55-58
   if (ntrans_out) *ntrans_out = c->ntrans;
   return GX_OK;
 }

@@ -34,7 +34,23 @@ struct LineStore {
   size_t cap = 0;
   double max_nu = 0.0;
   bool has_gamma = false;   // the last upload carried per-line gamma and n
+  // staging of unsorted uploads (raw copies, sort keys and values, sort scratch); grown on
+  // demand and kept, so repeated chunk uploads allocate nothing
+  double* s_nu = nullptr;
+  double* s_e = nullptr;
+  double* s_a = nullptr;
+  double* s_gam = nullptr;
+  double* s_n = nullptr;
+  int* s_g = nullptr;
+  uint32_t* s_idx_in = nullptr;
+  uint32_t* s_idx_out = nullptr;
+  size_t s_cap = 0;            // lines
+  bool s_voigt = false;        // s_gam / s_n allocated
+  void* s_tmp = nullptr;
+  size_t s_tmp_cap = 0;        // bytes
   int reserve(size_t n, bool voigt);
+  int reserve_staging(size_t n, bool voigt);
+  void release_staging();
   void release();
   int upload(size_t n, const double* e, const double* nu, const double* a, const int* g, const double* gam,
              const double* nexp, hipStream_t s);

@@ -320,6 +320,7 @@ int gx_lines_create(int device, size_t n, const double* energies, const double* 
   L->voigt = gamma_ref != nullptr;
   if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking))) { delete L; return hip_fail(e, "stream"); }
   int rc = L->ls.upload(n, energies, nu, aif, gns, gamma_ref, nexp, L->stream);
+  L->ls.release_staging();   // a resident list is uploaded once: give the sort staging back
   if (rc) { gx_lines_destroy(L); return rc; }
   L->h_nu.resize(n);
   if ((e = hipMemcpy(L->h_nu.data(), L->ls.nu, n * sizeof(double), hipMemcpyDeviceToHost))) {

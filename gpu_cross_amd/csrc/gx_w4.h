@@ -27,11 +27,24 @@ __device__ __forceinline__ double cdiv_re(cplx a, cplx b) {
   return (a.re * b.re + a.im * b.im) / (b.re * b.re + b.im * b.im);
 }
 
-// Region I (|x| + y >= 15) in closed real form, FP64.
+// num / den for den > 0 and normal: the hardware reciprocal refined by two Newton steps
+// (relative error squared each step), the quotient by one residual correction
+// q + (num - den q) / den; within an ulp or two of the IEEE quotient, without the scaling
+// steps an IEEE division spends on operands that cannot occur here.
+__device__ __forceinline__ double div_pos(double num, double den) {
+  double r = __builtin_amdgcn_rcp(den);
+  r = __builtin_fma(r, __builtin_fma(-den, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-den, r, 1.0), r);
+  const double q = num * r;
+  return __builtin_fma(__builtin_fma(-den, q, num), r, q);
+}
+
+// Region I (|x| + y >= 15) in closed real form, FP64.  The denominator is positive there
+// ((x^2 - y^2 - 1/2)^2 + 4 x^2 y^2 with |x| + y >= 15).
 __device__ __forceinline__ double w4_region1(double x, double y) {
   const double x2 = x * x, y2 = y * y;
   const double dr = 0.5 + y2 - x2;
-  return 0.5641896 * y * (0.5 + x2 + y2) / (dr * dr + 4.0 * x2 * y2);
+  return div_pos(0.5641896 * y * (0.5 + x2 + y2), dr * dr + 4.0 * x2 * y2);
 }
 
 // The region tests of W4, each operand rounded as the reference's plain C rounds it (no

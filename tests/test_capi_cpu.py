@@ -61,3 +61,30 @@ def test_grid_matches_reference_formula():
     assert d == 12000.0 / 1_200_000
     assert f[777] == 0.0 + 777.0 * d
     assert f[-1] == 12000.0 - d or abs(f[-1] - (12000.0 - d)) < 1e-9
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/BaseManager.h"), reason="needs the reference checkout")
+def test_cpp_manager_compiles_against_the_reference_basemanager(tmp_path, root):
+    """mi355x_manager.h built against the reference's own BaseManager.h (the route a
+    maintainer takes), every virtual overridden with its exact signature, linked to libgx.so."""
+    src = tmp_path / "use_manager.cpp"
+    src.write_text(
+        '#include "mi355x_manager.h"\n'
+        "BaseManager* make(ProfileType p) { return new Mi355xManager(p); }\n"
+        "void drive(BaseManager* m, double* f, double* I, double* e, double* nu, double* a, int* g,\n"
+        "           double* gam, double* n) {\n"
+        "  m->InitializeVectors(16); m->TransferFreq(f, I, 16);\n"
+        "  m->InitializeConstants(0.0, 1000.0, 1.0, 0.01, 17.03, 1.0, 500.0);\n"
+        "  m->TransferVectors(4, e, nu, a, g, gam, n); m->ExecuteCrossSection(16, 4, 0);\n"
+        "  m->TransferResults(f, I, 16); (void)m->ReadyForWork(); (void)m->GetNtrans(); m->Cleanup();\n"
+        "  delete static_cast<Mi355xManager*>(m);   // the reference's ~BaseManager is protected\n"
+        "}\n"
+        "int main() { return make(VOIGT) == nullptr; }\n")
+    exe = tmp_path / "use_manager"
+    cmd = ["g++", "-std=c++11", "-Wall", "-Werror=overloaded-virtual", '-DGX_BASE_MANAGER_HEADER="BaseManager.h"',
+           "-I/root/reference", "-I" + os.path.join(root, "gpu_cross_amd", "csrc"), "-I" + os.path.join(root, "include"),
+           str(src), "/root/reference/BaseManager.cpp", "-L" + os.path.join(root, "gpu_cross_amd"), "-lgx",
+           "-Wl,-rpath-link,/opt/rocm/lib",
+           "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
