@@ -239,21 +239,46 @@ int Derived::reserve(size_t n) {
   return GX_OK;
 }
 
-int Derived::reserve_mixed(size_t n, int npoints) {
-  if (n <= mcap && npoints <= mpoints) return GX_OK;
-  hipFree(m.amp); hipFree(m.y); hipFree(m.g); hipFree(m.core);
+int Derived::reserve_mixed(size_t n, int npoints, int ntemps) {
+  if (n <= mcap && npoints <= mpoints && ntemps <= mtemps) return GX_OK;
+  hipFree(m.amp); hipFree(m.g); hipFree(m.y); hipFree(m.core);
   m = MixedLineData{};
+  const size_t c = std::max<size_t>({n, mcap, 1});
+  const int np = std::max({npoints, mpoints, 1}), nt = std::max({ntemps, mtemps, 1});
   mcap = 0;
   mpoints = 0;
-  const size_t c = std::max<size_t>(n, 1), total = c * (size_t)std::max(npoints, 1);
+  mtemps = 0;
+  const size_t rows_t = c * (size_t)nt, rows_p = c * (size_t)np;
   hipError_t e;
-  if ((e = hipMalloc(&m.amp, total * 4)) || (e = hipMalloc(&m.y, total * 8)) || (e = hipMalloc(&m.g, total * 8)) ||
-      (e = hipMalloc(&m.core, total * sizeof(int2))))
+  if ((e = hipMalloc(&m.amp, rows_t * 4)) || (e = hipMalloc(&m.g, rows_t * 8)) || (e = hipMalloc(&m.y, rows_p * 8)) ||
+      (e = hipMalloc(&m.core, rows_p * sizeof(int2))))
     return hip_fail(e, "hipMalloc(mixed line data)");
   m.stride = c;
   mcap = c;
-  mpoints = std::max(npoints, 1);
+  mpoints = np;
+  mtemps = nt;
   return GX_OK;
+}
+
+int mixed_temperature_rows(const PointConsts* pcs, int n, int* row) {
+  // the same fields k_prep_mixed's same_t compares
+  int r = 0;
+  for (int p = 0; p < n; ++p) {
+    if (p > 0) {
+      const PointConsts& a = pcs[p], & b = pcs[p - 1];
+      const bool same = a.beta == b.beta && a.cmcoef == b.cmcoef && a.partition == b.partition &&
+                        a.dpwcoef == b.dpwcoef && a.ref_temp == b.ref_temp && a.temperature == b.temperature &&
+                        a.convention == b.convention;
+      if (!same) ++r;
+    }
+    if (row) row[p] = r;
+  }
+  return n > 0 ? r + 1 : 0;
+}
+
+MixedTask mixed_task(const MixedLineData& m, int point, int trow, const PointConsts& pc, double* out) {
+  const size_t ot = (size_t)trow * m.stride, op = (size_t)point * m.stride;
+  return MixedTask{m.amp + ot, m.g + ot, m.y + op, m.core + op, pc.dpwcoef, out};
 }
 
 int Derived::set_points(const PointConsts* pcs, int npts, hipStream_t s) {
@@ -277,10 +302,11 @@ void Derived::release() {
   hipFree(d.S); hipFree(d.g); hipFree(d.y); hipFree(d.ib); hipFree(d.ie);
   d = DerivedF64{};
   cap = 0;
-  hipFree(m.amp); hipFree(m.y); hipFree(m.g); hipFree(m.core);
+  hipFree(m.amp); hipFree(m.g); hipFree(m.y); hipFree(m.core);
   m = MixedLineData{};
   mcap = 0;
   mpoints = 0;
+  mtemps = 0;
   hipFree(d_pcs);
   d_pcs = nullptr;
   pcs_cap = 0;
@@ -630,9 +656,9 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   if ((rc = rng.reserve(nblocks))) return rc;
   hipError_t e;
   if (mixed) {
-    if ((rc = der.reserve_mixed((size_t)std::max(nlines, 1), 1))) return rc;
+    if ((rc = der.reserve_mixed((size_t)std::max(nlines, 1), 1, 1))) return rc;
     if ((rc = der.set_points(&pc, 1, s))) return rc;
-    MixedTask task{der.m.amp, der.m.y, der.m.g, der.m.core, pc.dpwcoef, intens};
+    MixedTask task = mixed_task(der.m, 0, 0, pc, intens);
     if ((rc = mix.prepare(&task, 1, nblocks, gv, nlines, s))) return rc;
   }
   if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");

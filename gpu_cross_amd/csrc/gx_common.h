@@ -63,13 +63,14 @@ struct Derived {
   DerivedF64 d{};
   size_t cap = 0;
   MixedLineData m{};
-  size_t mcap = 0;         // lines per point
-  int mpoints = 0;         // points
+  size_t mcap = 0;         // lines per row
+  int mpoints = 0;         // point rows (core ranges)
+  int mtemps = 0;          // temperature rows (amp, 1/alpha)
   PointConsts* d_pcs = nullptr;
   int pcs_cap = 0;
   std::vector<PointConsts> h_pcs;
   int reserve(size_t n);
-  int reserve_mixed(size_t n, int npoints);
+  int reserve_mixed(size_t n, int npoints, int ntemps);
   // upload per-point constants (waits for the stream only when they change)
   int set_points(const PointConsts* pcs, int npts, hipStream_t s);
   void release();
@@ -141,6 +142,13 @@ struct MixedRunner {
              hipEvent_t* ev = nullptr);
   void release();
 };
+
+// Temperature rows of the mixed line data for points pcs[0..n): a new row wherever a point's
+// temperature constants differ from the previous point's (as k_prep_mixed numbers them);
+// row[p] (may be NULL) gets point p's row.  Returns the number of rows.
+int mixed_temperature_rows(const PointConsts* pcs, int n, int* row);
+// Task of point `point` (temperature row trow) over the line data m, summing into out.
+MixedTask mixed_task(const MixedLineData& m, int point, int trow, const PointConsts& pc, double* out);
 
 // HBM of the far-field buffers of a mixed launch (ntasks points, nlines lines, nblocks blocks).
 size_t far_field_bytes(int ntasks, long long nlines, int nblocks);

@@ -65,16 +65,18 @@ struct DerivedF64 {
   int* ie;
 };
 
-// Per-point line data of the mixed-precision path, [npoints][stride].  1/alpha
-// and y stay in FP64 and the W4 core range of every line is fixed here, as an
-// integer bin range found with the FP64 path's own test |nu - f_j|/alpha + y < 15,
-// so every W4 region decision is the FP64 path's own.
+// Line data of the mixed-precision path.  What depends on the temperature only is kept
+// once per run of consecutive points at one temperature, [ntemps][stride]: S/alpha and
+// 1/alpha; per point, [npoints][stride]: y and the W4 core range, an integer bin range found
+// with the FP64 path's own test |nu - f_j|/alpha + y < 15.  1/alpha and y are the FP64 path's
+// own bits, so every W4 region decision is that path's.  16 B per line and point plus 12 B
+// per line and temperature.
 struct MixedLineData {
-  float* amp;     // S / alpha (FP32)
-  double* y;      // gamma_T / alpha
-  double* g;      // 1 / alpha = 1 / (nu * dpwcoef)
-  int2* core;     // [k0, k1): bins of the window where |x| + y < 15 (regions II-IV)
-  size_t stride;  // lines per point
+  float* amp;      // [ntemps][stride] S / alpha (FP32)
+  double* g;       // [ntemps][stride] 1 / alpha = 1 / (nu * dpwcoef)
+  double* y;       // [npoints][stride] gamma_T / alpha
+  int2* core;      // [npoints][stride] [k0, k1): bins of the window where |x| + y < 15 (regions II-IV)
+  size_t stride;   // lines per row
 };
 
 // Window rule shared by all points of a launch.
@@ -86,11 +88,11 @@ struct WindowArgs {
 
 // One (T,P) point of a mixed-precision launch: its line data and its grid.
 struct MixedTask {
-  const float* amp;
-  const double* y;
+  const float* amp;      // its temperature's rows of MixedLineData
   const double* g;
+  const double* y;       // its own rows
   const int2* core;
-  double dpwcoef;     // Doppler width per unit nu of this point
+  double dpwcoef;        // Doppler width per unit nu of this point
   double* out;
 };
 

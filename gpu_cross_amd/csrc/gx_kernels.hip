@@ -245,7 +245,7 @@ __device__ __forceinline__ bool same_t(const PointConsts& a, const PointConsts& 
          a.ref_temp == b.ref_temp && a.temperature == b.temperature && a.convention == b.convention;
 }
 
-// The pressure-dependent part and the outputs: y, amp, the core range.
+// The pressure-dependent part and the point's outputs: y and the core range.
 __device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts& pc, const PrepT& t, double nu_if,
                                                                double gamma, int ib, int ie, const GridView& gv,
                                                                double inv_dfreq, const MixedLineData& out, int i,
@@ -269,16 +269,22 @@ __device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts
     }
   }
   const size_t o = (size_t)p * out.stride + i;
-  out.amp[o] = (float)(t.S * g);
   out.y[o] = y;
-  out.g[o] = g;
   out.core[o] = make_int2(k0, k1);
   return (unsigned long long)(k1 - k0);
 }
 
+// The temperature row tq of one line.
+__device__ __forceinline__ void prep_mixed_store_t(const PrepT& t, const MixedLineData& out, int i, int tq) {
+  const size_t o = (size_t)tq * out.stride + i;
+  out.amp[o] = (float)(t.S * t.g);
+  out.g[o] = t.g;
+}
+
 // One thread per line over all npts points: the line's record is read once, and the
-// temperature-dependent part (two exp, one pow) is reused by consecutive points at the same
-// temperature (bit-identical: the same operations on the same operands).
+// temperature-dependent part (two exp, one pow) is computed and stored once per run of
+// consecutive points at one temperature (row tq counts the runs from 0, as
+// mixed_temperature_rows counts them on the host).
 __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, int npts, LinesView lv,
                                                     GridView gv, const int* __restrict__ IB,
                                                     const int* __restrict__ IE, MixedLineData out,
@@ -292,8 +298,13 @@ __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restric
     const int gi = lv.gns[i], ib = IB[i], ie = IE[i];
     const double inv_dfreq = 1.0 / gv.dfreq;
     PrepT t = prep_mixed_t(pcs[0], nu_if, a, gi, elow, nexp);
+    int tq = 0;
+    prep_mixed_store_t(t, out, i, 0);
     for (int p = 0; p < npts; ++p) {
-      if (p > 0 && !same_t(pcs[p], pcs[p - 1])) t = prep_mixed_t(pcs[p], nu_if, a, gi, elow, nexp);
+      if (p > 0 && !same_t(pcs[p], pcs[p - 1])) {
+        t = prep_mixed_t(pcs[p], nu_if, a, gi, elow, nexp);
+        prep_mixed_store_t(t, out, i, ++tq);
+      }
       w += prep_mixed_point(pcs[p], t, nu_if, gamma, ib, ie, gv, inv_dfreq, out, i, p);
     }
   }

@@ -74,7 +74,8 @@ struct Slot {
   int M = 0;
 
   size_t footprint() const {
-    return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 + der.mcap * (size_t)der.mpoints * 28 +
+    return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 +
+           der.mcap * ((size_t)der.mpoints * 16 + (size_t)der.mtemps * 12) +
            (size_t)rng.cap * sizeof(int2) + run.bytes();
   }
   void release_buffers() {
@@ -140,7 +141,7 @@ size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& b
   const size_t M = (size_t)r.npoints, B = batch.size();
   size_t need = std::max(sl.freq_cap, M) * 8 + std::max(sl.out_cap, M * B) * 8 + std::max(sl.win_cap, nlines) * 8;
   if (mixed_req(r)) {
-    need += std::max(sl.der.mcap * (size_t)sl.der.mpoints, nlines * B) * 28;
+    need += std::max(sl.der.mcap * ((size_t)sl.der.mpoints * 16 + (size_t)sl.der.mtemps * 12), nlines * B * 28);
     const size_t nb = (M + 1023) / 1024 + 1;
     need += nb * B * 16 * 1024 * sizeof(float2) / 4;   // partials, upper estimate
     need += (M + 4096) * B * 4 * sizeof(double);       // core-pass rows, upper estimate
@@ -221,7 +222,9 @@ int launch_batch(gx_server* S, Slot& sl) {
   const GridView gv{sl.freq, M, 0, M, r0.start_nu, d};
   const bool mixed = mixed_req(r0);
   if (mixed) {
-    if ((rc = sl.der.reserve_mixed((size_t)std::max(n, 1), B))) return rc;
+    if ((rc = sl.der.reserve_mixed((size_t)std::max(n, 1), B,
+                                   mixed_temperature_rows(sl.h_pcs.data(), B, nullptr))))
+      return rc;
     if ((rc = sl.der.set_points(sl.h_pcs.data(), B, s))) return rc;
   } else {
     if ((rc = sl.der.reserve((size_t)std::max(n, 1)))) return rc;
@@ -236,11 +239,10 @@ int launch_batch(gx_server* S, Slot& sl) {
     const int slots = sl.run.cus * mixed_occupancy(sl.run.R) * kWavesPerGroup;
     sl.run.forced_nseg = std::min(16, std::max(1, (4 * slots + nblocks - 1) / nblocks));
     sl.run.persistent = false;
+    std::vector<int> trow(B);
+    mixed_temperature_rows(sl.h_pcs.data(), B, trow.data());
     std::vector<MixedTask> tasks(B);
-    for (int k = 0; k < B; ++k)
-      tasks[k] = MixedTask{sl.der.m.amp + (size_t)k * sl.der.m.stride, sl.der.m.y + (size_t)k * sl.der.m.stride,
-                           sl.der.m.g + (size_t)k * sl.der.m.stride, sl.der.m.core + (size_t)k * sl.der.m.stride,
-                           sl.h_pcs[k].dpwcoef, sl.out + (size_t)k * M};
+    for (int k = 0; k < B; ++k) tasks[k] = mixed_task(sl.der.m, k, trow[k], sl.h_pcs[k], sl.out + (size_t)k * M);
     if ((rc = sl.run.prepare(tasks.data(), B, nblocks, gv, n, s))) return rc;
   }
   if ((e = hipEventRecord(sl.ev_start, s))) return hip_fail(e, "server: event");

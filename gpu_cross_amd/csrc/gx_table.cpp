@@ -172,12 +172,13 @@ int prepare_launches(gx_table* t) {
     if (rc0) return rc0;
     l.run.R = t->R;
     l.run.items_per_slot = t->ips;
+    // the launch's points are contiguous in the grouped constants k_prep_mixed reads
+    std::vector<int> trow(l.count);
+    mixed_temperature_rows(t->der.h_pcs.data() + g.first_const + l.first, l.count, trow.data());
     std::vector<MixedTask> tasks(l.count);
     for (int k = 0; k < l.count; ++k) {
       const int p = g.points[l.first + k];
-      const size_t off = (size_t)k * t->der.m.stride;
-      tasks[k] = MixedTask{t->der.m.amp + off, t->der.m.y + off, t->der.m.g + off, t->der.m.core + off,
-                           t->pcs[p].dpwcoef, t->d_out + (size_t)p * t->M};
+      tasks[k] = mixed_task(t->der.m, k, trow[k], t->pcs[p], t->d_out + (size_t)p * t->M);
     }
     const int nb = (t->M + mixed_bins_per_block(t->R) - 1) / mixed_bins_per_block(t->R);
     int rc = l.run.prepare(tasks.data(), l.count, nb, grid_view(t), (int)t->L->ls.n, t->stream);
@@ -427,20 +428,22 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
     size_t free_b = 0, total_b = 0;
     if ((e = hipMemGetInfo(&free_b, &total_b))) return bail(hip_fail(e, "hipMemGetInfo"));
     const size_t headroom = size_t(2) << 30;
-    // amp, y, 1/alpha, core range per line, the core pass's rows (from the same geometry the
-    // launch uses, at the point with the widest cores) and the far-field moments and block
+    // per line: y and the core range (16 B) per point and amp, 1/alpha (12 B) per temperature
+    // (shared by the points at it); the core pass's rows (from the same geometry
+    // the launch uses, at the point with the widest cores) and the far-field moments and block
     // plans; segment partials exist only while a launch has fewer items than ~4 per resident
     // wave, so they stay below 2 x 4 x 6144 items x 512 bins x 8 B = 200 MB, inside the headroom
     size_t rows = 0;
     for (int p = 0; p < npts; ++p) rows = std::max(rows, core_rows_bytes(grid_view(t), t->pcs[p].dpwcoef));
-    const size_t per_point = 28 * std::max<size_t>(n, 1) + rows + (size_t(64) << 10) +
-                             far_field_bytes(1, (long long)std::max<size_t>(n, 1), t->M / 64 + 1);
+    const int temps_all = mixed_temperature_rows(grouped.data(), npts, nullptr);
+    const size_t nl = std::max<size_t>(n, 1);
+    const size_t per_point = 16 * nl + (12 * nl * (size_t)temps_all + npts - 1) / npts + rows + (size_t(64) << 10) +
+                             far_field_bytes(1, (long long)nl, t->M / 64 + 1);
     const size_t avail = free_b > headroom ? free_b - headroom : 0;
     t->batch = (int)std::min<size_t>((size_t)npts, avail / per_point);
     if (t->batch < 1) return bail(fail(GX_OUT_OF_MEMORY, "table_create: not enough free HBM for one point"));
-    int rc = t->der.reserve_mixed(n, t->batch);
+    int rc = t->der.set_points(grouped.data(), npts, t->stream);
     if (rc) return bail(rc);
-    if ((rc = t->der.set_points(grouped.data(), npts, t->stream))) return bail(rc);
     // launches: batches of consecutive points inside each group
     for (int gi = 0; gi < (int)t->groups.size(); ++gi) {
       const int np = (int)t->groups[gi].points.size();
@@ -451,6 +454,13 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
         t->launches.back().count = std::min(t->batch, np - f0);
       }
     }
+    // temperature rows: the most any launch needs (each launch numbers its rows from 0)
+    int trows = 1;
+    for (const Launch& l : t->launches) {
+      const Group& g = t->groups[l.group];
+      trows = std::max(trows, mixed_temperature_rows(grouped.data() + g.first_const + l.first, l.count, nullptr));
+    }
+    if ((rc = t->der.reserve_mixed(n, t->batch, trows))) return bail(rc);
     if ((e = hipMalloc(&t->core_evals, sizeof(unsigned long long) * kEvalSlots)))
       return bail(hip_fail(e, "hipMalloc(core counters)"));
     for (Launch& l : t->launches)
