@@ -336,9 +336,11 @@ def bench_serving(args, wl):
         levels.append({"offered_load": load, "arrival_rate_rps": load * cap_rps, "requests": len(reqs),
                        "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms, "max_ms": st.max_ms,
                        "throughput_rps": len(reqs) / wall, "evals_per_s": st.evals / max(st.busy_s, 1e-9),
-                       "batches": st.batches, "failed": st.failed, "peak_hbm_bytes": st.peak_bytes})
+                       "batches": st.batches, "batched_requests": st.batched_requests,
+                       "shared_launch_share": st.shared_requests / max(st.batched_requests, 1),
+                       "failed": st.failed, "peak_hbm_bytes": st.peak_bytes})
         log(f"load {load}: p50 {st.p50_ms:.1f} ms p99 {st.p99_ms:.1f} ms, {len(reqs) / wall:.2f} req/s, "
-            f"{st.batches} batches")
+            f"{st.batches} batches, {st.shared_requests} requests in shared launches")
         keep = list(zip(reqs, results))
     # alone == in the mix, on a sample of the last level
     same = 0

@@ -87,7 +87,7 @@ class ServerSummary(ctypes.Structure):
                 ("failed", ctypes.c_longlong), ("batches", ctypes.c_longlong), ("batched_requests", ctypes.c_longlong),
                 ("evals", ctypes.c_longlong), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("busy_s", ctypes.c_double),
-                ("peak_bytes", ctypes.c_size_t)]
+                ("peak_bytes", ctypes.c_size_t), ("shared_requests", ctypes.c_longlong)]
 
 
 class OpacityInfo(ctypes.Structure):

@@ -158,6 +158,27 @@ struct MixedPlan {
   unsigned long long* far_evals;   // line-bin pairs taken from moments
 };
 
+// One request of a batched FP64 launch (launch_f64_batch): its constants, grid, line slice
+// [line_lo, line_lo + nlines) of the resident list, derived data (slice-local index) and sums.
+struct F64Task {
+  PointConsts pc;
+  GridView gv;
+  const double* nu;                // the whole sorted list
+  int line_lo;
+  int nlines;
+  DerivedF64 d;
+  double* out;
+  long long prep_first;            // first prep thread (a multiple of 256)
+  int first_block;                 // first of its bin blocks among all tasks' blocks
+  double zero_nu, x0;              // f64_zero_nu, f64_x0
+  unsigned long long* evals;       // kEvalSlots counters of its windows
+};
+
+// A chunk of work: lines [a, z) (absolute indices) over bin block `block` of task `task`.
+struct F64Item {
+  int task, block, a, z;
+};
+
 constexpr int kEvalSlots = 64;       // eval counters (spread to avoid one hot atomic)
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kTile = 256;           // lines staged in LDS per pass (FP64 kernels)
@@ -183,6 +204,14 @@ hipError_t launch_prep_mixed(const PointConsts* d_pcs, int npts, const LinesView
                              const int* ib, const int* ie, MixedLineData out, unsigned long long* core_evals,
                              hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
+// Batched FP64 path: prep of every task's slice (prep_threads threads), one workgroup per
+// item writing partial[item][kBinsF64], then out[j] += the partials of j's block in item
+// order (block_items[global block] = (first item, count)).
+hipError_t launch_f64_batch(const F64Task* d_tasks, int ntasks, long long prep_threads, const LinesView& all,
+                            const F64Item* d_items, int nitems, int nblocks_all, const int2* d_block_items,
+                            double* partial, hipStream_t s);
+double f64_zero_nu(const PointConsts& pc);   // nu == 0 stand-in of the FP64 sums
+double f64_x0(const PointConsts& pc, double dfreq);   // half-bin term of the erf-binned profiles
 hipError_t launch_grid(double* f, int n, double start, double dfreq, hipStream_t s);
 // R = bins per lane (4 or 8); a work item (one wave) covers mixed_bins_per_block(R) bins.
 // launch_accum_mixed: region I (and the segment reduction); launch_core_mixed: the W4 core pass.

@@ -99,12 +99,12 @@ __device__ __forceinline__ void block_count(unsigned long long w, unsigned long 
 // reference so that S, the Doppler width and the window edges round exactly
 // as in OpenMPManager.cpp:191-206; contraction into FMA is switched off here.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_line_prep(PointConsts pc, GridView gv, LinesView lv, DerivedF64 out,
-                                                   unsigned long long* __restrict__ evals) {
+// One line i of lv for the point pc: its derived data and window; returns ie - ib.
+__device__ __forceinline__ unsigned long long line_prep_one(const PointConsts& pc, const GridView& gv,
+                                                            const LinesView& lv, const DerivedF64& out, int i) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long w = 0;
-  if (i < lv.n && (pc.profile == kProfGaussian || pc.profile == kProfVoigtNoGamma)) {
+  if (pc.profile == kProfGaussian || pc.profile == kProfVoigtNoGamma) {
     // the CUDA manager's intensity step of both (cross_kernal.cu:471-496, 731-756): its cmcoef,
     // no ISQRTPI, nu == 0 gives S = 0; per-bin windows |nu - f| <= C
     const double nu = lv.nu[i];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_line_prep(PointConsts pc, GridView gv, 
     out.ib[i] = ib;
     out.ie[i] = ie;
     w = (unsigned long long)(ie - ib);
-  } else if (i < lv.n) {
+  } else {
     const bool voigt = pc.profile == 2;
     const bool gpu = pc.convention == 1;
     const double nu = lv.nu[i];
@@ -173,6 +173,13 @@ __global__ __launch_bounds__(256) void k_line_prep(PointConsts pc, GridView gv, 
     out.ie[i] = ie;
     w = (unsigned long long)(ie - ib);
   }
+  return w;
+}
+
+__global__ __launch_bounds__(256) void k_line_prep(PointConsts pc, GridView gv, LinesView lv, DerivedF64 out,
+                                                   unsigned long long* __restrict__ evals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long w = i < lv.n ? line_prep_one(pc, gv, lv, out, i) : 0ull;
   if (evals) block_count(w, evals);
 }
 
@@ -348,6 +355,61 @@ __global__ void k_block_ranges(GridView gv, const double* __restrict__ nu, int n
 // ---------------------------------------------------------------------------
 // FP64 accumulation: one bin per lane, 256 bins per workgroup.
 // ---------------------------------------------------------------------------
+// Sum of the lines [a, z) of the sorted list nu_s over bin j (frequency f) of one workgroup
+// in line order; the lines' derived data sit at index (line - dofs).  Lines are staged
+// through LDS in kTile-line tiles; every lane tests its own bin against each line's window.
+struct F64Tiles {
+  double nu[kTile], S[kTile], g[kTile], y[kTile];
+  int ib[kTile], ie[kTile];
+};
+
+template <int PROFILE>
+__device__ __forceinline__ double accum_f64_lines(F64Tiles& L, int j, double f, const double* __restrict__ nu_s,
+                                                  int a, int z, int dofs, const double* __restrict__ S,
+                                                  const double* __restrict__ G, const double* __restrict__ Y,
+                                                  const int* __restrict__ IB, const int* __restrict__ IE,
+                                                  double zero_nu, double x0) {
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int base = a; base < z; base += kTile) {
+    const int i = base + t;
+    if (i < z) {
+      const double v = nu_s[i];
+      L.nu[t] = v == 0.0 ? zero_nu : v;
+      L.S[t] = S[i - dofs];
+      L.g[t] = G[i - dofs];
+      if (PROFILE == kProfVoigt || PROFILE == kProfVoigtNoGamma) L.y[t] = Y[i - dofs];
+      L.ib[t] = IB[i - dofs];
+      L.ie[t] = IE[i - dofs];
+    }
+    __syncthreads();
+    const int cnt = min(kTile, z - base);
+    for (int k = 0; k < cnt; ++k) {
+      const int ib = L.ib[k], ie = L.ie[k];
+      if (j >= ib && j < ie) {
+        const double g = L.g[k];
+        if (PROFILE == kProfVoigt) {
+          const double x = fabs(L.nu[k] - f) * g;  // OpenMPManager.cpp:211-214
+          acc += L.S[k] * w4(x, L.y[k]) * g;
+        } else if (PROFILE == kProfGaussian) {
+          acc += plain_gauss_term(L.S[k], f - L.nu[k], g, x0);       // cross_kernal.cu:594-604
+        } else if (PROFILE == kProfVoigtNoGamma) {
+          acc += plain_nogamma_term(L.S[k], L.nu[k] - f, g, L.y[k]);  // cross_kernal.cu:884-904
+        } else {
+          const double d = fabs(L.nu[k] - f);      // OpenMPManager.cpp:165-171
+          const double xm = g * (d - x0);
+          if (xm < 6.0) {                           // beyond, erf(xp) == erf(xm) == 1 exactly
+            const double xp = g * (d + x0);
+            acc += L.S[k] * (erf(xp) - erf(xm));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
 template <int PROFILE>
 __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double* __restrict__ nu_s,
                                                       const double* __restrict__ S, const double* __restrict__ G,
@@ -355,52 +417,93 @@ __global__ __launch_bounds__(kBlock) void k_accum_f64(GridView gv, const double*
                                                       const int* __restrict__ IE, const int2* __restrict__ ranges,
                                                       int nblocks, double* __restrict__ intens, double zero_nu,
                                                       double x0) {
-  __shared__ double s_nu[kTile], s_S[kTile], s_g[kTile], s_y[kTile];
-  __shared__ int s_ib[kTile], s_ie[kTile];
-  const int t = threadIdx.x;
+  __shared__ F64Tiles L;
   const int b = blockIdx.x;
-  const int j = gv.start_idx + b * kBinsF64 + t;
+  const int j = gv.start_idx + b * kBinsF64 + threadIdx.x;
   const bool active = j < gv.start_idx + gv.points;
   const double f = active ? gv.freq[j] : 0.0;
   const int2 r = ranges[b];
-  double acc = 0.0;
-  for (int base = r.x; base < r.y; base += kTile) {
-    const int i = base + t;
-    if (i < r.y) {
-      const double v = nu_s[i];
-      s_nu[t] = v == 0.0 ? zero_nu : v;
-      s_S[t] = S[i];
-      s_g[t] = G[i];
-      if (PROFILE == kProfVoigt || PROFILE == kProfVoigtNoGamma) s_y[t] = Y[i];
-      s_ib[t] = IB[i];
-      s_ie[t] = IE[i];
-    }
-    __syncthreads();
-    const int cnt = min(kTile, r.y - base);
-    for (int k = 0; k < cnt; ++k) {
-      const int ib = s_ib[k], ie = s_ie[k];
-      if (j >= ib && j < ie) {
-        const double g = s_g[k];
-        if (PROFILE == kProfVoigt) {
-          const double x = fabs(s_nu[k] - f) * g;  // OpenMPManager.cpp:211-214
-          acc += s_S[k] * w4(x, s_y[k]) * g;
-        } else if (PROFILE == kProfGaussian) {
-          acc += plain_gauss_term(s_S[k], f - s_nu[k], g, x0);       // cross_kernal.cu:594-604
-        } else if (PROFILE == kProfVoigtNoGamma) {
-          acc += plain_nogamma_term(s_S[k], s_nu[k] - f, g, s_y[k]);  // cross_kernal.cu:884-904
-        } else {
-          const double d = fabs(s_nu[k] - f);      // OpenMPManager.cpp:165-171
-          const double xm = g * (d - x0);
-          if (xm < 6.0) {                           // beyond, erf(xp) == erf(xm) == 1 exactly
-            const double xp = g * (d + x0);
-            acc += s_S[k] * (erf(xp) - erf(xm));
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
+  const double acc = accum_f64_lines<PROFILE>(L, j, f, nu_s, r.x, r.y, 0, S, G, Y, IB, IE, zero_nu, x0);
   if (active) intens[j] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched FP64 path: several requests, each with its own grid, constants and line slice,
+// in three launches.  Prep: one thread per line of every request's slice (each request's
+// threads start on a workgroup boundary, so its windows are counted into its own counters).
+// Accumulation: one workgroup per item (request, bin block, chunk of lines), writing its
+// partial sums; reduction: per bin, the partials of its block in item order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_line_prep_batch(const F64Task* __restrict__ tasks, int ntasks, LinesView all) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int lo = 0, hi = ntasks - 1;                     // last task with prep_first <= this workgroup's first thread
+  const long long t0 = (long long)blockIdx.x * blockDim.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].prep_first <= t0) lo = mid; else hi = mid - 1;
+  }
+  const F64Task& tk = tasks[lo];
+  const long long i = t - tk.prep_first;
+  unsigned long long w = 0;
+  if (i >= 0 && i < tk.nlines) {
+    const LinesView lv{all.nu + tk.line_lo, all.elow + tk.line_lo, all.aif + tk.line_lo, all.gns + tk.line_lo,
+                       all.gamma ? all.gamma + tk.line_lo : nullptr, all.nexp ? all.nexp + tk.line_lo : nullptr,
+                       tk.nlines};
+    w = line_prep_one(tk.pc, tk.gv, lv, tk.d, (int)i);
+  }
+  block_count(w, tasks[lo].evals);
+}
+
+__global__ __launch_bounds__(kBlock) void k_accum_f64_items(const F64Task* __restrict__ tasks,
+                                                            const F64Item* __restrict__ items,
+                                                            double* __restrict__ partial) {
+  __shared__ F64Tiles L;
+  const F64Item it = items[blockIdx.x];
+  const F64Task& tk = tasks[it.task];
+  const GridView& gv = tk.gv;
+  const int j = gv.start_idx + it.block * kBinsF64 + threadIdx.x;
+  const bool active = j < gv.start_idx + gv.points;
+  const double f = active ? gv.freq[j] : 0.0;
+  const DerivedF64& d = tk.d;
+  double acc;
+  switch (tk.pc.profile) {
+    case kProfVoigt:
+      acc = accum_f64_lines<kProfVoigt>(L, j, f, tk.nu, it.a, it.z, tk.line_lo, d.S, d.g, d.y, d.ib, d.ie,
+                                        tk.zero_nu, tk.x0);
+      break;
+    case kProfDoppler:
+      acc = accum_f64_lines<kProfDoppler>(L, j, f, tk.nu, it.a, it.z, tk.line_lo, d.S, d.g, d.y, d.ib, d.ie,
+                                          tk.zero_nu, tk.x0);
+      break;
+    case kProfGaussian:
+      acc = accum_f64_lines<kProfGaussian>(L, j, f, tk.nu, it.a, it.z, tk.line_lo, d.S, d.g, d.y, d.ib, d.ie,
+                                           tk.zero_nu, tk.x0);
+      break;
+    default:
+      acc = accum_f64_lines<kProfVoigtNoGamma>(L, j, f, tk.nu, it.a, it.z, tk.line_lo, d.S, d.g, d.y, d.ib, d.ie,
+                                               tk.zero_nu, tk.x0);
+      break;
+  }
+  partial[(size_t)blockIdx.x * kBinsF64 + threadIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kBinsF64) void k_reduce_f64_items(const F64Task* __restrict__ tasks, int ntasks,
+                                                               const int2* __restrict__ block_items,
+                                                               const double* __restrict__ partial) {
+  const int gb = blockIdx.x;                       // block over all tasks
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].first_block <= gb) lo = mid; else hi = mid - 1;
+  }
+  const F64Task& tk = tasks[lo];
+  const int b = gb - tk.first_block;
+  const int j = tk.gv.start_idx + b * kBinsF64 + threadIdx.x;
+  if (j >= tk.gv.start_idx + tk.gv.points) return;
+  const int2 bi = block_items[gb];
+  double sum = 0.0;
+  for (int c = 0; c < bi.y; ++c) sum += partial[(size_t)(bi.x + c) * kBinsF64 + threadIdx.x];
+  tk.out[j] += sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -965,6 +1068,37 @@ hipError_t launch_accum_f64(const PointConsts& pc, const GridView& gv, const dou
     k_accum_f64<1><<<nblocks, kBlock, 0, s>>>(gv, nu_sorted, d.S, d.g, d.y, d.ib, d.ie, ranges, nblocks, intens,
                                               zero_nu, gv.dfreq * 0.5);
   return hipGetLastError();
+}
+
+hipError_t launch_f64_batch(const F64Task* d_tasks, int ntasks, long long prep_threads, const LinesView& all,
+                            const F64Item* d_items, int nitems, int nblocks_all, const int2* d_block_items,
+                            double* partial, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipError_t e;
+  if (prep_threads > 0) {
+    k_line_prep_batch<<<(unsigned)((prep_threads + 255) / 256), 256, 0, s>>>(d_tasks, ntasks, all);
+    if ((e = hipGetLastError())) return e;
+  }
+  if (nitems > 0) {
+    k_accum_f64_items<<<nitems, kBlock, 0, s>>>(d_tasks, d_items, partial);
+    if ((e = hipGetLastError())) return e;
+  }
+  if (nblocks_all > 0) {
+    k_reduce_f64_items<<<nblocks_all, kBinsF64, 0, s>>>(d_tasks, ntasks, d_block_items, partial);
+    if ((e = hipGetLastError())) return e;
+  }
+  return hipSuccess;
+}
+
+double f64_zero_nu(const PointConsts& pc) {
+  const bool zero_to_tiny = pc.profile == kProfVoigt || (pc.profile == kProfDoppler && pc.convention == 0);
+  return zero_to_tiny ? 1e-6 : 0.0;
+}
+
+double f64_x0(const PointConsts& pc, double dfreq) {
+  if (pc.profile == kProfGaussian) return pc.dpwcoef * dfreq * 0.5;
+  if (pc.profile == kProfDoppler) return dfreq * 0.5;
+  return 0.0;
 }
 
 hipError_t launch_windows(const WindowArgs& wa, const GridView& gv, const double* nu, int n, int* ib, int* ie,

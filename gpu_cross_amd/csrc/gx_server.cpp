@@ -67,29 +67,58 @@ struct Slot {
   int* ie = nullptr;
   size_t win_cap = 0;
   unsigned long long* evals = nullptr;
+  size_t evals_cap = 0;        // counters
   Derived der;
   Ranges rng;
   MixedRunner run;
   Clock::time_point t_launch;
-  int M = 0;
+  // where each request's grid sits in out, and whether the batch ran the batched FP64 path
+  // (one eval counter set per request) or the mixed path (one set for the batch)
+  std::vector<size_t> out_off;
+  bool f64 = false;
+  // batched FP64 path: tasks, work items and the block -> items table (the host copies stay
+  // until the batch retires, since the uploads read them), their device copies, item partials
+  std::vector<F64Task> h_t64;
+  std::vector<F64Item> h_items;
+  std::vector<int2> h_bitems;
+  F64Task* d_t64 = nullptr;
+  size_t t64_cap = 0;
+  F64Item* d_items = nullptr;
+  size_t items_cap = 0;
+  int2* d_bitems = nullptr;
+  size_t bitems_cap = 0;
+  double* partial = nullptr;
+  size_t partial_cap = 0;      // doubles
 
   size_t footprint() const {
     return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 +
            der.mcap * ((size_t)der.mpoints * 16 + (size_t)der.mtemps * 12) +
-           (size_t)rng.cap * sizeof(int2) + run.bytes();
+           (size_t)rng.cap * sizeof(int2) + run.bytes() + evals_cap * 8 + t64_cap * sizeof(F64Task) +
+           items_cap * sizeof(F64Item) + bitems_cap * sizeof(int2) + partial_cap * 8;
   }
   void release_buffers() {
     (void)hipFree(freq); (void)hipFree(out); (void)hipFree(ib); (void)hipFree(ie);
     freq = out = nullptr;
     ib = ie = nullptr;
     freq_cap = out_cap = win_cap = 0;
+    (void)hipFree(d_t64); (void)hipFree(d_items); (void)hipFree(d_bitems); (void)hipFree(partial);
+    d_t64 = nullptr;
+    d_items = nullptr;
+    d_bitems = nullptr;
+    partial = nullptr;
+    t64_cap = items_cap = bitems_cap = partial_cap = 0;
     der.release();
     rng.release();
     run.release();
   }
 };
 
+bool fp64_req(const gx_request& r) { return r.precision == GX_PREC_FP64; }
+
+// Requests that can share a launch: any two FP64 requests (the batched FP64 path gives each
+// its own grid, constants and line slice); mixed-precision requests on one grid and cutoff.
 bool compatible(const gx_request& a, const gx_request& b) {
+  if (fp64_req(a) && fp64_req(b)) return true;
   return a.profile == b.profile && a.precision == b.precision && a.start_nu == b.start_nu && a.end_nu == b.end_nu &&
          a.npoints == b.npoints && reference_cutoff(a.profile, a.pressure, a.cutoff) ==
          reference_cutoff(b.profile, b.pressure, b.cutoff) && a.doppler_widths == b.doppler_widths;
@@ -127,7 +156,7 @@ struct gx_server {
   std::thread worker;
 
   // statistics (under mu)
-  long long submitted = 0, completed = 0, cancelled = 0, failed = 0, batches = 0, batched = 0, evals = 0;
+  long long submitted = 0, completed = 0, cancelled = 0, failed = 0, batches = 0, batched = 0, evals = 0, shared = 0;
   std::vector<double> latencies_ms;
   bool started = false;
   Clock::time_point t_first, t_last;
@@ -136,8 +165,11 @@ struct gx_server {
 
 namespace {
 
+size_t f64_batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& batch);
+
 size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& batch, size_t nlines) {
   const gx_request& r = batch[0]->r;
+  if (fp64_req(r)) return f64_batch_need(S, sl, batch);
   const size_t M = (size_t)r.npoints, B = batch.size();
   size_t need = std::max(sl.freq_cap, M) * 8 + std::max(sl.out_cap, M * B) * 8 + std::max(sl.win_cap, nlines) * 8;
   if (mixed_req(r)) {
@@ -188,7 +220,149 @@ void line_slice(const gx_lines* L, double start, double end, double cmax, double
   hi = std::upper_bound(L->h_nu.begin(), L->h_nu.end(), b) - L->h_nu.begin();
 }
 
+// Grid spacing, own widest window and own line slice of one request.
+struct ReqGeo {
+  double d, cut;
+  size_t lo, hi;
+};
+
+ReqGeo request_geo(const gx_server* S, const gx_request& q, const PointConsts& pc) {
+  ReqGeo g{};
+  g.d = (q.end_nu - q.start_nu) / double(q.npoints);     // BaseProfile.cpp:10
+  g.cut = max_cutoff(pc, S->L->ls.max_nu, g.d);
+  line_slice(S->L, q.start_nu, q.start_nu + q.npoints * g.d, g.cut, g.d, g.lo, g.hi);
+  return g;
+}
+
+// Device bytes of a batched FP64 launch: grids and sums, derived data of every slice, the
+// item partials (at most two items per block plus one per kFp64Chunk lines of the blocks'
+// ranges; a line lies in the ranges of at most (2 cut + 3 dnu) / (256 dnu) + 2 blocks).
+constexpr long long kFp64Chunk = 4096;    // smallest chunk of lines per item
+size_t f64_batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& batch) {
+  size_t bins = 0, lines = 0, items = 0, blocks = 0;
+  for (const Req* q : batch) {
+    const ReqGeo g = request_geo(S, q->r, request_consts(S, q->r));
+    const size_t M = (size_t)q->r.npoints, n = g.hi - g.lo, nb = (M + kBinsF64 - 1) / kBinsF64;
+    const double per_line = (2.0 * g.cut + 3.0 * g.d) / (kBinsF64 * g.d) + 2.0;
+    bins += M;
+    lines += n;
+    blocks += nb;
+    items += 2 * nb + (size_t)std::ceil((double)n * per_line / (double)kFp64Chunk);
+  }
+  const size_t B = batch.size();
+  return std::max(sl.freq_cap, bins) * 8 + std::max(sl.out_cap, bins) * 8 + std::max(sl.der.cap, lines) * 32 +
+         std::max(sl.partial_cap, items * kBinsF64) * 8 + items * sizeof(F64Item) + blocks * sizeof(int2) +
+         B * (sizeof(F64Task) + kEvalSlots * 8);
+}
+
+template <class T>
+int grow_copy(T*& d, size_t& cap, const std::vector<T>& h, hipStream_t s, const char* what) {
+  int rc = grow(d, cap, h.size(), what);
+  if (rc || h.empty()) return rc;
+  hipError_t e = hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s);
+  return e ? hip_fail(e, what) : GX_OK;
+}
+
+// FP64 requests of any grids in one batch: each request's grid, constants and line slice
+// (from its own widest window), its bin blocks' line ranges from the host copy of the sorted
+// centres, and work items of lines [a, z) cut at multiples of its chunk K (absolute line
+// indices; K from its own ranges), so its sums are the same alone and in any batch.
+int launch_f64_batch_server(gx_server* S, Slot& sl) {
+  const int B = (int)sl.reqs.size();
+  const LineStore& ls = S->L->ls;
+  const std::vector<double>& hnu = S->L->h_nu;
+  hipStream_t s = sl.stream;
+  hipError_t e;
+  int rc;
+  sl.f64 = true;
+  sl.h_pcs.resize(B);
+  sl.out_off.assign(B, 0);
+  std::vector<ReqGeo> geo(B);
+  size_t bins = 0, lines = 0;
+  for (int k = 0; k < B; ++k) {
+    const gx_request& q = sl.reqs[k]->r;
+    sl.h_pcs[k] = request_consts(S, q);
+    geo[k] = request_geo(S, q, sl.h_pcs[k]);
+    sl.out_off[k] = bins;
+    bins += (size_t)q.npoints;
+    lines += geo[k].hi - geo[k].lo;
+  }
+  if ((rc = grow(sl.freq, sl.freq_cap, bins, "server: grids")) || (rc = grow(sl.out, sl.out_cap, bins, "server: sums")) ||
+      (rc = grow(sl.evals, sl.evals_cap, (size_t)B * kEvalSlots, "server: counters")))
+    return rc;
+  if ((rc = sl.der.reserve(std::max<size_t>(lines, 1)))) return rc;
+  sl.h_t64.assign(B, F64Task{});
+  sl.h_items.clear();
+  sl.h_bitems.clear();
+  size_t dofs = 0;
+  long long prep = 0;
+  int gblock = 0;
+  for (int k = 0; k < B; ++k) {
+    const gx_request& q = sl.reqs[k]->r;
+    const ReqGeo& g = geo[k];
+    const int M = q.npoints;
+    F64Task& t = sl.h_t64[k];
+    t.pc = sl.h_pcs[k];
+    t.gv = GridView{sl.freq + sl.out_off[k], M, 0, M, q.start_nu, g.d};
+    t.nu = ls.nu;
+    t.line_lo = (int)g.lo;
+    t.nlines = (int)(g.hi - g.lo);
+    t.d = DerivedF64{sl.der.d.S + dofs, sl.der.d.g + dofs, sl.der.d.y + dofs, sl.der.d.ib + dofs, sl.der.d.ie + dofs};
+    t.out = sl.out + sl.out_off[k];
+    t.prep_first = prep;
+    t.first_block = gblock;
+    t.zero_nu = f64_zero_nu(t.pc);
+    t.x0 = f64_x0(t.pc, g.d);
+    t.evals = sl.evals + (size_t)k * kEvalSlots;
+    dofs += (size_t)t.nlines;
+    prep += ((long long)t.nlines + 255) / 256 * 256;
+    // block ranges (the lines that can reach a block, k_block_ranges' rule) on the host
+    const int nb = (M + kBinsF64 - 1) / kBinsF64;
+    std::vector<int2> rg(nb);
+    long long total = 0;
+    for (int b = 0; b < nb; ++b) {
+      const int j0 = b * kBinsF64, j1 = std::min(j0 + kBinsF64, M);
+      const double lo_nu = q.start_nu + (double)(j0 - 2) * g.d - g.cut;
+      const double hi_nu = q.start_nu + (double)(j1 + 1) * g.d + g.cut;
+      const int a = (int)(std::lower_bound(hnu.begin() + g.lo, hnu.begin() + g.hi, lo_nu) - hnu.begin());
+      const int z = (int)(std::upper_bound(hnu.begin() + a, hnu.begin() + g.hi, hi_nu) - hnu.begin());
+      rg[b] = make_int2(a, z);
+      total += z - a;
+    }
+    const long long K = std::max(kFp64Chunk, (total / 2048 + kFp64Chunk - 1) / kFp64Chunk * kFp64Chunk);
+    for (int b = 0; b < nb; ++b) {
+      const int first = (int)sl.h_items.size();
+      const long long a = rg[b].x, z = rg[b].y;
+      if (a < z)
+        for (long long c = a / K; c <= (z - 1) / K; ++c)
+          sl.h_items.push_back(F64Item{k, b, (int)std::max(a, c * K), (int)std::min(z, (c + 1) * K)});
+      sl.h_bitems.push_back(make_int2(first, (int)sl.h_items.size() - first));
+    }
+    gblock += nb;
+  }
+  const int nitems = (int)sl.h_items.size();
+  if ((rc = grow(sl.partial, sl.partial_cap, (size_t)std::max(nitems, 1) * kBinsF64, "server: partials")) ||
+      (rc = grow_copy(sl.d_t64, sl.t64_cap, sl.h_t64, s, "server: tasks")) ||
+      (rc = grow_copy(sl.d_items, sl.items_cap, sl.h_items, s, "server: items")) ||
+      (rc = grow_copy(sl.d_bitems, sl.bitems_cap, sl.h_bitems, s, "server: blocks")))
+    return rc;
+  if ((e = hipEventRecord(sl.ev_start, s))) return hip_fail(e, "server: event");
+  for (int k = 0; k < B; ++k) {
+    const gx_request& q = sl.reqs[k]->r;
+    if ((e = launch_grid(sl.freq + sl.out_off[k], q.npoints, q.start_nu, geo[k].d, s))) return hip_fail(e, "server: grid");
+  }
+  if ((e = launch_fill(sl.out, bins, 0.0, s)) ||
+      (e = launch_fill(reinterpret_cast<double*>(sl.evals), (size_t)B * kEvalSlots, 0.0, s)))
+    return hip_fail(e, "server: fill");
+  const LinesView all{ls.nu, ls.elow, ls.aif, ls.gns, ls.gam, ls.nexp, (int)ls.n};
+  if ((e = launch_f64_batch(sl.d_t64, B, prep, all, sl.d_items, nitems, gblock, sl.d_bitems, sl.partial, s)))
+    return hip_fail(e, "server: FP64 batch");
+  if ((e = hipEventRecord(sl.ev_done, s))) return hip_fail(e, "server: event");
+  return GX_OK;
+}
+
 int launch_batch(gx_server* S, Slot& sl) {
+  if (fp64_req(sl.reqs[0]->r)) return launch_f64_batch_server(S, sl);
   const gx_request& r0 = sl.reqs[0]->r;
   const int M = r0.npoints;
   const int B = (int)sl.reqs.size();
@@ -218,24 +392,20 @@ int launch_batch(gx_server* S, Slot& sl) {
       return hip_fail(e, "server: windows");
     sl.win_cap = n;
   }
-  if (!sl.evals && (e = hipMalloc(&sl.evals, sizeof(unsigned long long) * kEvalSlots))) return hip_fail(e, "server");
+  if ((rc = grow(sl.evals, sl.evals_cap, (size_t)kEvalSlots, "server: counters"))) return rc;
+  sl.f64 = false;
+  sl.out_off.resize(B);
+  for (int k = 0; k < B; ++k) sl.out_off[k] = (size_t)k * M;
   const GridView gv{sl.freq, M, 0, M, r0.start_nu, d};
-  const bool mixed = mixed_req(r0);
-  if (mixed) {
-    if ((rc = sl.der.reserve_mixed((size_t)std::max(n, 1), B,
-                                   mixed_temperature_rows(sl.h_pcs.data(), B, nullptr))))
-      return rc;
-    if ((rc = sl.der.set_points(sl.h_pcs.data(), B, s))) return rc;
-  } else {
-    if ((rc = sl.der.reserve((size_t)std::max(n, 1)))) return rc;
-  }
+  if ((rc = sl.der.reserve_mixed((size_t)std::max(n, 1), B, mixed_temperature_rows(sl.h_pcs.data(), B, nullptr))))
+    return rc;
+  if ((rc = sl.der.set_points(sl.h_pcs.data(), B, s))) return rc;
   // launch shape from the request's own grid only (alone == in a mix)
-  if (mixed && (rc = sl.run.init())) return rc;   // fixes R (GX_MIXED_R) before the block size is used
-  const int bpb = mixed ? sl.run.bins_per_block() : kBinsF64;
+  if ((rc = sl.run.init())) return rc;   // fixes R (GX_MIXED_R) before the block size is used
+  const int bpb = sl.run.bins_per_block();
   const int nblocks = (M + bpb - 1) / bpb;
   if ((rc = sl.rng.reserve(nblocks))) return rc;
-  if (mixed) {
-    if ((rc = sl.run.init())) return rc;
+  {
     const int slots = sl.run.cus * mixed_occupancy(sl.run.R) * kWavesPerGroup;
     sl.run.forced_nseg = std::min(16, std::max(1, (4 * slots + nblocks - 1) / nblocks));
     sl.run.persistent = false;
@@ -251,23 +421,13 @@ int launch_batch(gx_server* S, Slot& sl) {
   if ((e = launch_fill(reinterpret_cast<double*>(sl.evals), kEvalSlots, 0.0, s))) return hip_fail(e, "server: fill");
   if (n > 0) {
     const WindowArgs wa{pc0.cutoff, pc0.convention, 1};
-    if (mixed) {
-      if ((e = launch_windows(wa, gv, lv.nu, n, sl.ib, sl.ie, sl.evals, s))) return hip_fail(e, "server: windows");
-      if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, gv, sl.ib, sl.ie, sl.der.m, nullptr, s))) return hip_fail(e, "server: prep");
-      if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
-      if ((rc = sl.run.launch(gv, lv.nu, sl.rng.r, sl.ib, sl.ie, s))) return rc;
-    } else {
-      if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
-      for (int k = 0; k < B; ++k) {
-        if ((e = launch_line_prep_f64(sl.h_pcs[k], gv, lv, sl.der.d, k == 0 ? sl.evals : nullptr, s)))
-          return hip_fail(e, "server: prep");
-        if ((e = launch_accum_f64(sl.h_pcs[k], gv, lv.nu, sl.der.d, sl.rng.r, nblocks, sl.out + (size_t)k * M, s)))
-          return hip_fail(e, "server: accumulate");
-      }
-    }
+    if ((e = launch_windows(wa, gv, lv.nu, n, sl.ib, sl.ie, sl.evals, s))) return hip_fail(e, "server: windows");
+    if ((e = launch_prep_mixed(sl.der.d_pcs, B, lv, gv, sl.ib, sl.ie, sl.der.m, nullptr, s)))
+      return hip_fail(e, "server: prep");
+    if ((e = launch_block_ranges(gv, lv.nu, n, cmax, bpb, nblocks, sl.rng.r, s))) return hip_fail(e, "server: ranges");
+    if ((rc = sl.run.launch(gv, lv.nu, sl.rng.r, sl.ib, sl.ie, s))) return rc;
   }
   if ((e = hipEventRecord(sl.ev_done, s))) return hip_fail(e, "server: event");
-  sl.M = M;
   return GX_OK;
 }
 
@@ -275,17 +435,20 @@ void retire(gx_server* S, Slot& sl) {
   const int B = (int)sl.reqs.size();
   float ms = 0;
   (void)hipEventElapsedTime(&ms, sl.ev_start, sl.ev_done);
-  unsigned long long slots[kEvalSlots] = {0};
-  hipError_t e = hipMemcpy(slots, sl.evals, sizeof(slots), hipMemcpyDeviceToHost);
-  unsigned long long ev = 0;
-  for (auto v : slots) ev += v;
+  // evaluation counters: one set per request (batched FP64 path) or one for the batch
+  const int nsets = sl.f64 ? B : 1;
+  std::vector<unsigned long long> slots((size_t)nsets * kEvalSlots, 0ull);
+  hipError_t e = hipMemcpy(slots.data(), sl.evals, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost);
+  std::vector<long long> ev(B, 0);
+  for (int k = 0; k < B; ++k)
+    for (int q = 0; q < kEvalSlots; ++q) ev[k] += (long long)slots[(size_t)(sl.f64 ? k : 0) * kEvalSlots + q];
   std::vector<std::vector<double>> results(B);
   bool ok = e == hipSuccess;
   for (int k = 0; k < B && ok; ++k) {
     if (sl.reqs[k]->cancel) continue;
-    results[k].resize(sl.M);
-    if (hipMemcpy(results[k].data(), sl.out + (size_t)k * sl.M, sizeof(double) * sl.M, hipMemcpyDeviceToHost))
-      ok = false;
+    const int M = sl.reqs[k]->r.npoints;
+    results[k].resize(M);
+    if (hipMemcpy(results[k].data(), sl.out + sl.out_off[k], sizeof(double) * M, hipMemcpyDeviceToHost)) ok = false;
   }
   const auto now = Clock::now();
   std::lock_guard<std::mutex> lk(S->mu);
@@ -302,10 +465,10 @@ void retire(gx_server* S, Slot& sl) {
       ++S->failed;
     } else {
       q->host.swap(results[k]);
-      q->evals = (long long)ev;
+      q->evals = ev[k];
       q->state = GX_REQ_DONE;
       ++S->completed;
-      S->evals += (long long)ev;
+      S->evals += ev[k];
       S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - q->t_submit).count());
     }
   }
@@ -403,6 +566,7 @@ void scheduler(gx_server* S) {
         }
         ++S->batches;
         S->batched += (long long)batch.size();
+        if (batch.size() > 1) S->shared += (long long)batch.size();
         S->peak_bytes = std::max(S->peak_bytes, others + need);
       }
       sl.reqs = batch;
@@ -585,6 +749,7 @@ int gx_server_stats(gx_server* S, gx_server_summary* out) {
   out->batched_requests = S->batched;
   out->evals = S->evals;
   out->peak_bytes = S->peak_bytes;
+  out->shared_requests = S->shared;
   if (S->started) out->busy_s = std::chrono::duration<double>(S->t_last - S->t_first).count();
   std::vector<double> v = S->latencies_ms;
   if (!v.empty()) {

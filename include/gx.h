@@ -254,6 +254,7 @@ typedef struct gx_server_summary {
     double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> result ready, completed requests */
     double busy_s;          /* wall time from first admission to last completion */
     size_t peak_bytes;      /* largest in-flight device footprint */
+    long long shared_requests;   /* requests that ran in a batch of two or more */
 } gx_server_summary;
 
 int gx_server_create(gx_lines *lines, const gx_server_opts *opts, gx_server **out);

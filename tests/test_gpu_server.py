@@ -100,6 +100,21 @@ def test_cancel_and_budget_rejection(world):
             srv.result(rid)
 
 
+def _occupy(srv, ll, profile=capi.VOIGT, grid=(100.0, 11000.0, 400_000)):
+    """A larger request launched first, so the one slot is busy while the requests that follow
+    queue (and batch); returns once it runs -- the scheduler otherwise may launch the first of
+    the others alone, or batch them with it."""
+    import time
+    Q = po.partition(ll.state_e, ll.state_g, 1000.0)
+    prec = capi.PREC_MIXED if profile == capi.VOIGT else capi.PREC_FP64
+    rid = srv.submit(1000.0, 1.0, Q, *grid, profile=profile, precision=prec, cutoff=0.0)
+    t0 = time.time()
+    while srv.poll(rid) == capi.REQ_QUEUED:
+        assert time.time() - t0 < 60.0
+        time.sleep(1e-4)
+    return rid
+
+
 def test_doppler_batch_of_two_temperatures_near_the_top(world):
     """Per-line Doppler windows (k alpha + dfreq) widen with temperature: a batch of a cold
     and a hot request on one grid near the top of the list takes the hot one's line slice,
@@ -117,14 +132,16 @@ def test_doppler_batch_of_two_temperatures_near_the_top(world):
                  precision=capi.PREC_FP64, cutoff=0.0, doppler_widths=3.0) for T in (300.0, 3000.0)]
     try:
         with Server(lines, 17.03, max_inflight=1, max_batch=4) as srv:
+            first = _occupy(srv, ll, capi.DOPPLER, (11000.0, 12000.0, 1_000_000))
             ids = [submit(srv, r) for r in reqs]
             mix = []
             for rid in ids:
                 assert srv.wait(rid, 120.0) == capi.REQ_DONE
                 mix.append(srv.result(rid)[0])
+            assert srv.wait(first, 120.0) == capi.REQ_DONE
             st = srv.stats()
             print(f"doppler batch: {st.batches} batches for {st.completed} requests", flush=True)
-            assert st.batches == 1                 # both in one launch
+            assert st.batches == 2 and st.shared_requests == 2     # both Doppler requests in one launch
         with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
             for r, m in zip(reqs, mix):
                 rid = submit(srv, r)
@@ -179,3 +196,90 @@ def test_failed_and_cancelled_requests_are_released(world):
         with pytest.raises(capi.GxError) as ei:           # released: the id is gone
             srv.poll(rid)
         assert ei.value.status == capi.GX_NOT_FOUND
+
+
+def _fp64_requests(ll):
+    """FP64 requests on six different grids: Doppler (reference 100 cm^-1 window, and per-line
+    windows) and FP64 Voigt, coarse grids whose blocks each take thousands of lines."""
+    Q = lambda T: po.partition(ll.state_e, ll.state_g, T)
+    specs = [(capi.DOPPLER, 300.0, 1.0, (1000.0, 1100.0, 1000), 0.0, 0.0),
+             (capi.DOPPLER, 2500.0, 1.0, (4000.0, 4006.0, 3000), 0.0, 25.0),
+             (capi.VOIGT, 900.0, 0.3, (2000.0, 2050.0, 500), 10.0, 0.0),
+             (capi.DOPPLER, 1200.0, 1.0, (7000.0, 7400.0, 40000), 0.0, 25.0),
+             (capi.VOIGT, 2200.0, 3.0, (5000.0, 5002.0, 2000), 25.0, 0.0),
+             (capi.DOPPLER, 700.0, 1.0, (9000.0, 9500.0, 500), 0.0, 0.0)]
+    return [dict(T=T, P=P, Q=Q(T), grid=g, profile=prof, precision=capi.PREC_FP64, cutoff=cut, doppler_widths=dw)
+            for prof, T, P, g, cut, dw in specs]
+
+
+def _oracle_check(ll, r, m, tol):
+    start, end, npts = r["grid"]
+    freq, d = make_grid(start, end, npts)
+    voigt = r["profile"] == capi.VOIGT
+    cut = r["cutoff"] if voigt else 100.0
+    c = po.consts(r["profile"], r["T"], r["Q"], d, 17.03, r["P"], 500.0, cutoff=cut,
+                  doppler_widths=r["doppler_widths"])
+    sel = np.nonzero((ll.nu > start - cut - 1.0) & (ll.nu < end + cut + 1.0))[0]
+    L = ll.subset(sel)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, L.gamma_at(r["P"]) if voigt else None,
+                           L.nexp if voigt else None, freq, npts, 0, 8)
+    mask = ref >= 1e-6 * ref.max()
+    err = float(np.max(np.abs(m[mask] - ref[mask]) / ref[mask]))
+    assert err <= tol, (r["profile"], r["grid"], err)
+    return err
+
+
+def test_fp64_requests_of_different_grids_share_a_launch(world):
+    ll, lines = world
+    reqs = _fp64_requests(ll)
+    with Server(lines, 17.03, max_inflight=1, max_batch=8) as srv:
+        first = _occupy(srv, ll)
+        ids = [submit(srv, r) for r in reqs]
+        mix = []
+        for rid in ids:
+            assert srv.wait(rid, 120.0) == capi.REQ_DONE
+            out, t = srv.result(rid)
+            mix.append((out, t.evals))
+        assert srv.wait(first, 120.0) == capi.REQ_DONE
+        st = srv.stats()
+    print(f"fp64 grids: {st.batches} batches for {st.completed} requests", flush=True)
+    assert st.batches == 2 and st.shared_requests == len(reqs)     # the six in one launch
+    with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
+        for r, (m, ev) in zip(reqs, mix):
+            rid = submit(srv, r)
+            assert srv.wait(rid, 120.0) == capi.REQ_DONE
+            alone, t = srv.result(rid)
+            assert np.array_equal(alone, m), r["grid"]
+            assert t.evals == ev                        # each request counts its own windows
+    for r, (m, _) in zip(reqs, mix):
+        err = _oracle_check(ll, r, m, 1e-10)
+        print(f"  profile={r['profile']} grid={r['grid']}: max rel err {err:.2e}", flush=True)
+
+
+def test_poisson_arrivals_alone_equals_mix(world):
+    """Seeded Poisson arrivals while batches run on three slots: every request's grid equals
+    its grid alone, bit for bit, and FP64 requests share launches."""
+    import time
+    ll, lines = world
+    rng = np.random.default_rng(17)
+    base = requests(ll) + _fp64_requests(ll)
+    order = rng.permutation(len(base))
+    gaps = rng.exponential(0.004, len(base))
+    with Server(lines, 17.03, max_inflight=3, max_batch=8) as srv:
+        ids = []
+        for k, gap in zip(order, gaps):
+            time.sleep(gap)
+            ids.append((k, submit(srv, base[k])))
+        mix = {}
+        for k, rid in ids:
+            assert srv.wait(rid, 120.0) == capi.REQ_DONE
+            mix[k] = srv.result(rid)[0]
+        st = srv.stats()
+    print(f"poisson: {st.batches} batches for {st.completed} requests, p50 {st.p50_ms:.1f} ms "
+          f"p99 {st.p99_ms:.1f} ms", flush=True)
+    assert st.completed == len(base) and st.failed == 0
+    with Server(lines, 17.03, max_inflight=1, max_batch=1) as srv:
+        for k, r in enumerate(base):
+            rid = submit(srv, r)
+            assert srv.wait(rid, 120.0) == capi.REQ_DONE
+            assert np.array_equal(srv.result(rid)[0], mix[k]), (k, r["grid"], r["profile"], r["precision"])
