@@ -306,21 +306,22 @@ def bench_serving(args, wl):
                                        profile=r["profile"], precision=r["precision"], cutoff=r["cutoff"],
                                        doppler_widths=r["doppler_widths"])
     rng = np.random.default_rng(55)
-    # capacity: a closed-loop burst of requests
-    calib = serving_requests(rng, 24, ll, Qfun)
+    # capacity: the same request set as the open-loop levels, submitted as one closed-loop burst
+    # (request costs span orders of magnitude, so a smaller or different set misjudges it)
+    reqs = serving_requests(rng, args.requests, ll, Qfun)
     with Server(lines, ll.mass, max_inflight=4, max_batch=16) as srv:
         t1 = time.perf_counter()
-        ids = [submit(srv, r) for r in calib]
+        ids = [submit(srv, r) for r in reqs]
         for rid in ids:
             srv.wait(rid)
             srv.result(rid)
-        cap_rps = len(calib) / (time.perf_counter() - t1)
+        cap_rps = len(reqs) / (time.perf_counter() - t1)
     log(f"closed-loop capacity ~{cap_rps:.2f} requests/s")
     levels = []
     keep = []
-    for load in (0.5, 0.9):
-        reqs = serving_requests(rng, args.requests, ll, Qfun)
-        gaps = rng.exponential(1.0 / (load * cap_rps), len(reqs))
+    # the 90 % level twice, with its own arrival seeds, so its p99 is not one draw
+    for load, seed in ((0.5, 1), (0.9, 2), (0.9, 3)):
+        gaps = np.random.default_rng(seed).exponential(1.0 / (load * cap_rps), len(reqs))
         with Server(lines, ll.mass, max_inflight=4, max_batch=16) as srv:
             t1 = time.perf_counter()
             ids = []
@@ -333,7 +334,8 @@ def bench_serving(args, wl):
                 results.append(srv.result(rid)[0])
             wall = time.perf_counter() - t1
             st = srv.stats()
-        levels.append({"offered_load": load, "arrival_rate_rps": load * cap_rps, "requests": len(reqs),
+        levels.append({"offered_load": load, "arrival_seed": seed, "arrival_rate_rps": load * cap_rps,
+                       "requests": len(reqs),
                        "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms, "max_ms": st.max_ms,
                        "throughput_rps": len(reqs) / wall, "evals_per_s": st.evals / max(st.busy_s, 1e-9),
                        "batches": st.batches, "batched_requests": st.batched_requests,
@@ -351,7 +353,7 @@ def bench_serving(args, wl):
             srv.wait(rid)
             same += int(np.array_equal(srv.result(rid)[0], mixed))
     lines.close()
-    top = levels[-1]
+    top = max(levels[1:], key=lambda lv: lv["p99_ms"])
     res = {"metric": "serving p50/p99 latency (ms) at 90% offered load; throughput", "value": top["p99_ms"],
            "unit": "ms (p99)", "n_gpus": 1, "steps": args.requests, "warmup": 0,
            "ms_per_step": top["mean_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -360,6 +362,8 @@ def bench_serving(args, wl):
                                   "T U[200,3000] K, P log-U[1e-5,100] atm, width log-U[10,12000] cm^-1, "
                                   "resolution log-U[1e-3,0.1] cm^-1 (<= 1e7 bins), Voigt cutoff 25 cm^-1 mixed, "
                                   "Doppler 25 HWHM FP64", "closed_loop_capacity_rps": cap_rps},
+           "value_definition": "the larger p99 of two 90 % offered-load runs (arrival seeds 2 and 3) over the "
+                               "same request set; offered rate = 0.9 x the closed-loop capacity of that set",
            "levels": levels, "alone_equals_mix": f"{same}/{len(sample)} sampled requests bit-identical"}
     print(json.dumps(res), flush=True)
 

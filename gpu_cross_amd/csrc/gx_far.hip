@@ -35,6 +35,8 @@
 //
 // Cluster record (kFarRec doubles): radius, max 1/g, min y, lowest and highest gamma,
 // gamma_c (0 for real centres), then Q_1..Q_24 (real centres) or (Re, Im) of M_0..M_23.
+// Each task's records are stored field-major (FarRec): field f of cluster k at
+// f * nclus + k, so threads on consecutive clusters read and write consecutive doubles.
 #include <hip/hip_runtime.h>
 
 #include "gx_internal.h"
@@ -49,6 +51,15 @@ constexpr double kPoleShift = 0.70710678118654752440;   // a = 1/sqrt(2)
 constexpr double kKappa = 0.5 * 0.5641896;             // c / 2, c of humlic's region I
 
 __device__ __forceinline__ double nu_if(double v) { return v == 0.0 ? 1e-6 : v; }
+
+// One cluster's record inside a task's field-major record array: rc[f] is field f.
+template <class T>
+struct FarRec {
+  T* p;            // field 0 of the cluster
+  size_t stride;   // clusters of the task (the distance between fields)
+  __device__ __forceinline__ T& operator[](int f) const { return p[(size_t)f * stride]; }
+};
+using FarRecC = FarRec<const double>;
 
 // Cluster geometry, for every level: centre = midpoint of its lines' nu range.
 __global__ void k_far_geom(MixedPlan P, double4* __restrict__ geom) {
@@ -149,18 +160,9 @@ constexpr int kLeafTasks = 8;
 // sum of its terms' magnitudes, and the series weighs moment k by at most eta^(k+1), so the
 // far sums keep ~1e-7 relative accuracy (the weights are 0.28 S, S/alpha being FP32 in the
 // mixed path already).  The records stay FP64 for the tree and the plans.
-__global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
-  const int lane = threadIdx.x & 63;
-  const int leaf = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (leaf >= P.far_n[0]) return;
-  const int l = leaf * kFarLeaf + lane;
-  const bool act = l < P.nlines;
-  const double c = P.far_geom[leaf].x;
-  const double nu = act ? nu_if(P.nu[l]) : c;                  // read once for kLeafTasks points
-  const int t1 = min(P.ntasks, (int)(blockIdx.y + 1) * kLeafTasks);
-  for (int task = blockIdx.y * kLeafTasks; task < t1; ++task) {
-    const MixedTask tk = P.tasks[task];
-    const bool cplx = P.far_kind[task] != 0;
+__device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int l, bool act, double nu, double c,
+                                            int lane, FarRec<double> rec) {
+  {
     double y = 0.0;
     float amp = 0.0f;
     if (act) {
@@ -188,7 +190,6 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
     const double ir = 1.0 / r;
     const float kap = act ? (float)(kKappa * (double)amp * al) : 0.0f;   // amp = S g
     const float wi = (float)(wim * ir);
-    double* rec = P.far_rec + ((size_t)task * P.far_nclus + leaf) * kFarRec;
     if (lane == 0) {
       rec[0] = r;
       rec[1] = amax;
@@ -245,6 +246,44 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   }
 }
 
+// Leaf records of four consecutive leaves (one per wave) are staged in LDS and written field
+// by field, four doubles per field; workgroups are mapped so that each XCD takes a contiguous
+// run of leaves (workgroups go to the XCDs in turn), and the 128-byte lines of a field, shared
+// by 16 leaves, fill in one L2.  Against one store per lane and field straight to HBM this
+// saves the per-store address arithmetic of a VALU-bound kernel and the partial-line
+// write-backs from four L2s (54 ms against 61.6 ms per 48-point table).
+__global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
+  __shared__ double s_rec[4][kFarRec + 1];   // +1: the rows start in different banks
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int per = gridDim.x / 8;              // the launch pads gridDim.x to a multiple of 8
+  const int grp = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int leaf0 = grp * 4;
+  const int nleaf = min(4, P.far_n[0] - leaf0);
+  if (nleaf <= 0) return;                     // the whole workgroup
+  const int leaf = leaf0 + wave;
+  const bool valid = wave < nleaf;
+  const int l = leaf * kFarLeaf + lane;
+  const bool act = valid && l < P.nlines;
+  const double c = valid ? P.far_geom[leaf].x : 0.0;
+  const double nu = act ? nu_if(P.nu[l]) : c;  // read once for kLeafTasks points
+  const size_t ns = (size_t)P.far_nclus;
+  const int t1 = min(P.ntasks, (int)(blockIdx.y + 1) * kLeafTasks);
+  for (int task = blockIdx.y * kLeafTasks; task < t1; ++task) {
+    const MixedTask tk = P.tasks[task];
+    const bool cplx = P.far_kind[task] != 0;
+    if (valid) leaf_record(tk, cplx, l, act, nu, c, lane, FarRec<double>{&s_rec[wave][0], 1});
+    __syncthreads();
+    const int nf = cplx ? kFarRec : kFarHead + kFarP;
+    double* dst = P.far_rec + (size_t)task * ns * kFarRec + leaf0;
+    for (int idx = threadIdx.x; idx < nf * 4; idx += 256) {
+      const int f = idx >> 2, lf = idx & 3;
+      if (lf < nleaf) dst[(size_t)f * ns + lf] = s_rec[lf][f];
+    }
+    __syncthreads();
+  }
+}
+
 // Parents of level lev from their two children.  Real centres: shift a = (c_child - c) / r
 // and scale b = r_child / r, Q_k = sum_m C(k, m) a^(k-m) b^m Q_m(child).  Complex centres:
 // the same with a complex shift, M_k = sum_m C(k, m) a^(k-m) b^m M_m(child), formed by
@@ -252,14 +291,15 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
 __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.far_n[lev]) return;
-  const size_t tbase = (size_t)blockIdx.y * P.far_nclus;
+  double* const tb = P.far_rec + (size_t)blockIdx.y * P.far_nclus * kFarRec;   // the task's records
+  const size_t ns = (size_t)P.far_nclus;
   const bool cplx = P.far_kind[blockIdx.y] != 0;
   const int self = P.far_off[lev] + i;
   const double c = P.far_geom[self].x;
   const int nch = min(2, P.far_n[lev - 1] - 2 * i);
   double amax = 0.0, ymin = 1e300, glo = 1e300, ghi = -1e300;
   for (int h = 0; h < nch; ++h) {
-    const double* cr = P.far_rec + (tbase + P.far_off[lev - 1] + 2 * i + h) * kFarRec;
+    const FarRecC cr{tb + P.far_off[lev - 1] + 2 * i + h, ns};
     amax = fmax(amax, cr[1]);
     ymin = fmin(ymin, cr[2]);
     glo = fmin(glo, cr[3]);
@@ -269,12 +309,12 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
   double r = 0.0;
   for (int h = 0; h < nch; ++h) {
     const int ch = P.far_off[lev - 1] + 2 * i + h;
-    const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+    const FarRecC cr{tb + ch, ns};
     const double dx = P.far_geom[ch].x - c, dy = cr[5] - gc;
     r = fmax(r, (cplx ? sqrt(dx * dx + dy * dy) : fabs(dx)) + cr[0]);
   }
   if (!(r > 0.0)) r = 1.0;
-  double* rec = P.far_rec + (tbase + self) * kFarRec;
+  const FarRec<double> rec{tb + self, ns};
   rec[0] = r;
   rec[1] = amax;
   rec[2] = ymin;
@@ -287,7 +327,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
     for (int h = 0; h < nch; ++h) {
       const int ch = P.far_off[lev - 1] + 2 * i + h;
-      const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+      const FarRecC cr{tb + ch, ns};
       const double a = (P.far_geom[ch].x - c) / r, b = cr[0] / r;
       double pa[kFarP], qb[kFarP];                             // a^j, b^m Q_m
       pa[0] = 1.0;
@@ -319,7 +359,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     for (int k = 0; k < kFarP; ++k) qr[k] = qi[k] = 0.0;
     for (int h = 0; h < nch; ++h) {
       const int ch = P.far_off[lev - 1] + 2 * i + h;
-      const double* cr = P.far_rec + (tbase + ch) * kFarRec;
+      const FarRecC cr{tb + ch, ns};
       const double ar = (P.far_geom[ch].x - c) / r, ai = (cr[5] - gc) / r, b = cr[0] / r;
       double vr[kFarP], vi[kFarP];
       double bm = 1.0;
@@ -409,7 +449,7 @@ __device__ __forceinline__ bool series_ok(double r, double gamma_c, double gamma
 // Whether cluster (geometry gm, record rc) serves every bin of [f_lo, f_hi]: no line of it
 // has its W4 core there (its lowest y is >= 15, or its lines are 15 Doppler widths away)
 // and its series converges there within the truncation bound of its kind.
-__device__ __forceinline__ bool serves(const double4 gm, const double* __restrict__ rc, bool cplx, double f_lo,
+__device__ __forceinline__ bool serves(const double4 gm, FarRecC rc, bool cplx, double f_lo,
                                        double f_hi) {
 #pragma clang fp contract(off)
   const double dist = fmax(fmax(f_lo - gm.x, gm.x - f_hi), 0.0);      // centre to the bins
@@ -419,7 +459,7 @@ __device__ __forceinline__ bool serves(const double4 gm, const double* __restric
 }
 
 // Sum of a cluster (record rc, centre nu_c) at bin f.
-__device__ __forceinline__ double cluster_sum(const double* __restrict__ rc, double nu_c, bool cplx, double f) {
+__device__ __forceinline__ double cluster_sum(FarRecC rc, double nu_c, bool cplx, double f) {
   if (!cplx) {
     const double id = 1.0 / (f - nu_c);
     const double t = rc[0] * id;
@@ -468,7 +508,7 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
     const bool cplx = P.far_kind[task] != 0;
     auto usable = [&](int lev, int idx) -> bool {
       const int k = P.far_off[lev] + idx;
-      return serves(P.far_geom[k], recs + (size_t)k * kFarRec, cplx, f_lo, f_hi);
+      return serves(P.far_geom[k], FarRecC{recs + k, (size_t)P.far_nclus}, cplx, f_lo, f_hi);
     };
     int i = leafL;
     if (leafL < leafR) {
@@ -544,7 +584,7 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
     auto leaves_usable = [&](int a, int z) -> bool {       // every lane calls it together
       bool ok = true;
       for (int leaf = a / kFarLeaf + lane; ok && leaf * kFarLeaf < z; leaf += 64)
-        ok = serves(P.far_geom[leaf], recs + (size_t)leaf * kFarRec, cplx, f_lo, f_hi);
+        ok = serves(P.far_geom[leaf], FarRecC{recs + leaf, (size_t)P.far_nclus}, cplx, f_lo, f_hi);
       return __ballot(!ok) == 0ull;
     };
     if (em1 > em0 && leaves_usable(em0, em1)) {
@@ -559,11 +599,16 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
   if (lane == 0) P.edge_rng[slot] = er;
 }
 
-// Far-field sums of one (task, block) per wave, added to the task's grid.
+// Far-field sums of one (task, block) per wave, added to the task's grid.  A cluster's record
+// is fetched by one load over the lanes (lane i takes field i; the fields lie in different
+// rows of the field-major array), staged in LDS and read back as broadcasts; the next
+// cluster's fetch is in flight while the current one is summed.
 template <int R>
 __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
+  __shared__ double s_rc[4][kFarRec];
   const int lane = threadIdx.x & 63;
-  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x * 4 + wave;
   if (b >= P.nblocks) return;
   const int task = blockIdx.y;
   const size_t slot = (size_t)task * P.nblocks + b;
@@ -574,6 +619,7 @@ __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
   const int w0 = gv.start_idx + b * 64 * R;
   const int* list = P.far_list + slot * kFarList;
   const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  const size_t ns = (size_t)P.far_nclus;
   double f[R], acc[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -581,18 +627,32 @@ __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
     acc[q] = 0.0;
   }
   const bool cplx = P.far_kind[task] != 0;
-  for (int e = 0; e < cnt; ++e) {
+  const int nf = cplx ? kFarRec : kFarHead + kFarP;
+  auto cluster = [&](int e) {
     const int ent = list[e];
-    const int lev = ent >> 24, idx = ent & 0xffffff;
-    const int k = P.far_off[lev] + idx;
-    const double c = P.far_geom[k].x;
-    const double* rc = recs + (size_t)k * kFarRec;
+    return P.far_off[ent >> 24] + (ent & 0xffffff);
+  };
+  double* my = s_rc[wave];
+  int k = cluster(0);
+  double v = lane < nf ? recs[(size_t)lane * ns + k] : 0.0;
+  double c = P.far_geom[k].x;
+  for (int e = 0; e < cnt; ++e) {
+    wave_lds_handover();                     // the previous cluster's reads are done
+    if (lane < nf) my[lane] = v;
+    wave_lds_handover();
+    const double cc = c;
+    if (e + 1 < cnt) {                       // next record in flight
+      k = cluster(e + 1);
+      v = lane < nf ? recs[(size_t)lane * ns + k] : 0.0;
+      c = P.far_geom[k].x;
+    }
+    const FarRecC rc{my, 1};
     if (cplx) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, c, true, f[q]);
+      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, cc, true, f[q]);
     } else {
 #pragma unroll
-      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, c, false, f[q]);
+      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, cc, false, f[q]);
     }
   }
 #pragma unroll
@@ -646,7 +706,7 @@ __device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask&
       if (i + (1 << lev) > B) continue;
       const int k = P.far_off[lev] + (i >> lev);
       const double4 gm = P.far_geom[k];
-      const double* rc = recs + (size_t)k * kFarRec;
+      const FarRecC rc{recs + k, (size_t)P.far_nclus};
       if (!serves(gm, rc, cplx, f, f)) continue;
       s += cluster_sum(rc, gm.x, cplx, f);
       i += 1 << lev;
@@ -681,7 +741,7 @@ struct RangeMoments {
 // Leaf moments (record rc, centre nu_l) shifted to the centre cE + i gE with radius rE,
 // the first kRangeP orders added to m.
 template <bool CPLX>
-__device__ __forceinline__ void leaf_to_range(const double* __restrict__ rc, double nu_l, double cE, double gE,
+__device__ __forceinline__ void leaf_to_range(FarRecC rc, double nu_l, double cE, double gE,
                                               double rE, double (&m)[RangeMoments<CPLX>::N]) {
   const double b = rc[0] / rE;
   if (!CPLX) {
@@ -828,7 +888,7 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
   double glo = 1e300, ghi = -1e300;                            // Lorentz widths: complex centres only
   if (CPLX) {
     for (int L = U0 + tid; L < U1; L += NB) {
-      const double* rc = recs + (size_t)L * kFarRec;
+      const FarRecC rc{recs + L, (size_t)P.far_nclus};
       glo = fmin(glo, rc[3]);
       ghi = fmax(ghi, rc[4]);
     }
@@ -838,7 +898,7 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
   const double gE = CPLX ? 0.5 * (glo + ghi) : 0.0;
   double rE = 0.0;
   for (int L = U0 + tid; L < U1; L += NB) {
-    const double* rc = recs + (size_t)L * kFarRec;
+    const FarRecC rc{recs + L, (size_t)P.far_nclus};
     const double dx = P.far_geom[L].x - cE, dy = rc[5] - gE;
     rE = fmax(rE, (CPLX ? sqrt(dx * dx + dy * dy) : fabs(dx)) + rc[0]);
   }
@@ -850,7 +910,8 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
   double m[RangeMoments<CPLX>::N];
 #pragma unroll
   for (int i = 0; i < RangeMoments<CPLX>::N; ++i) m[i] = 0.0;
-  for (int L = own0; L < own1; ++L) leaf_to_range<CPLX>(recs + (size_t)L * kFarRec, P.far_geom[L].x, cE, gE, rE, m);
+  for (int L = own0; L < own1; ++L)
+    leaf_to_range<CPLX>(FarRecC{recs + L, (size_t)P.far_nclus}, P.far_geom[L].x, cE, gE, rE, m);
   block_scan<RangeMoments<CPLX>::N, NW, SUFFIX>(m, s_tot);
   return range_value<CPLX>(m, cE, gE, rE, f);
 }
@@ -907,7 +968,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   k_far_geom<<<(plan.far_nclus + 255) / 256, 256, 0, s>>>(plan, plan.far_geom);
   k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
   k_far_kind<<<plan.ntasks, 256, 0, s>>>(plan);
-  k_far_leaf<<<dim3((plan.far_n[0] + 3) / 4, (plan.ntasks + kLeafTasks - 1) / kLeafTasks), 256, 0, s>>>(plan);
+  k_far_leaf<<<dim3(((plan.far_n[0] + 3) / 4 + 7) / 8 * 8, (plan.ntasks + kLeafTasks - 1) / kLeafTasks), 256, 0, s>>>(plan);
   for (int lev = 1; lev < plan.far_nlev; ++lev)
     k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
   k_bin_first<<<(plan.gv.points + 1 + 255) / 256, 256, 0, s>>>(plan);

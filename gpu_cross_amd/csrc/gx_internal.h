@@ -9,6 +9,14 @@
 
 namespace gx {
 
+// LDS written by the lanes of one wave and read back by the same wave: LDS
+// instructions of a wave complete in order; this keeps the compiler from
+// moving accesses across the hand-over.
+__device__ __forceinline__ void wave_lds_handover() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Constants of one (T, P) point, derived on the host in FP64 exactly as the
 // reference's managers derive them (OpenMPManager.cpp:57-67,191-192).
 struct PointConsts {
@@ -145,7 +153,7 @@ struct MixedPlan {
   int far_n[kFarMaxLevels];     // clusters of each level
   int far_nclus;           // clusters of all levels
   double4* far_geom;       // [far_nclus]: centre, lowest nu, highest nu (shared by the tasks)
-  double* far_rec;         // [ntasks][far_nclus][kFarRec]: head and moments of each cluster
+  double* far_rec;         // [ntasks][kFarRec][far_nclus]: head and moments of each cluster, field-major
   int* far_kind;           // [ntasks]: 0 real-centred moments, 1 complex-centred
   int4* far_rng;           // [ntasks][nblocks]: far line ranges [x, y) and [z, w) of each block
   int4* edge_rng;          // [ntasks][nblocks]: window-end lines from moments: [x, y) end inside the

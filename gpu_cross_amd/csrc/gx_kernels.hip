@@ -587,14 +587,6 @@ __device__ __forceinline__ float region1(float x, float h, float k, float a, flo
 
 __global__ void k_reset_counter(unsigned* c) { *c = 0u; }
 
-// LDS written by the lanes of one wave and read back by the same wave: LDS
-// instructions of a wave complete in order; this keeps the compiler from
-// moving accesses across the hand-over.
-__device__ __forceinline__ void wave_lds_handover() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 template <int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ? 6 : 3))) void k_accum_voigt_mixed(MixedPlan P) {
   constexpr int kBins = MixedShape<R>::kBins;
@@ -851,13 +843,15 @@ __global__ void k_reduce_segments(MixedPlan P, int kBinsMixed) {
 // W4 core pass: regions II-IV of every (line, bin) pair with bin in the line's
 // core [k0, k1), added to the grids after the region-I kernel.
 //
-// Lines are taken in nu order, 64 per wave, one line per lane; the wave walks
-// the offsets k = bin - centre bin together, so all lanes sit at about the
-// same x and take the same W4 region (the branches stay coherent).  A line's
-// value at offset k belongs to bin c + k, c its centre bin.  Lanes with equal
-// c form contiguous segments (nu is sorted); a segment of one lane adds its
-// value straight into the workgroup's row of bin sums, a longer one is first
-// summed across lanes by a fixed DPP tree.  A workgroup (one wave) owns the
+// Lines are taken in nu order, 64 RL per wave, RL consecutive lines per lane
+// (RL = 4 where lines are dense, else 1); the wave walks the offsets
+// k = bin - base bin together, so all lanes sit at about the same x and take
+// the same W4 region (the branches stay coherent).  A lane's value at offset k
+// is the sum over its lines at bin c + k, c its base bin (its first line's
+// centre bin).  Lanes with equal c form contiguous segments (nu is sorted); a
+// segment of one lane adds its value straight into the workgroup's row of bin
+// sums, a longer one is first summed across lanes by a fixed DPP tree.  A
+// workgroup (one wave) owns the
 // lines of P.cpw consecutive centre bins and a row of FP64 sums over
 // the bins they can reach; k_core_gather adds the rows covering a bin in row
 // order.  Every sum is in a fixed order: the grids are deterministic.
@@ -916,11 +910,108 @@ __device__ __forceinline__ float seg_scan(float v, float m1, float m2, float m4,
   return v;
 }
 
-// kFreqLds: grid values of the row staged in LDS, interleaved with the sums (entry i of the
-// row at s_core[2i], its frequency at s_core[2i + 1]); else the sums alone and the
-// frequencies read from HBM.
+// One line of a core chunk, held in registers for the offset loop.
+struct CoreLine {
+  double nu, g, y;
+  float amp;
+  int lo, hi;          // core offsets from the lane's base bin, inside the row; empty when lo >= hi
+  W4Region2 r2;
+};
+
+// Lines per lane where a workgroup's lines are dense (at least kCoreDenseLines per centre
+// bin on average): a lane's kCoreLanesLines consecutive lines then share their centre bin,
+// or nearly, and the scan and the LDS add run once for all of them.
+constexpr int kCoreLaneLines = 4;
+constexpr int kCoreDenseLines = 2 * kCoreLaneLines;
+
+// The chunks [l0, l1) of a core workgroup with RL lines per lane (lane L takes lines
+// chunk + RL L + r).  The lane's base bin cb is its first line's centre bin; all its lines
+// are evaluated at bin cb + k (each with its own x, so a line whose centre bin differs from cb
+// only shifts its core range).  kFreqLds: grid values of the row staged in LDS, interleaved
+// with the sums (entry i at s_core[2i], its frequency at s_core[2i + 1]); else the sums alone
+// and the frequencies read from HBM.
+template <bool kFreqLds, int RL>
+__device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask& tk, double* s_core, int base,
+                                            int l0, int l1) {
+  constexpr int kStride = kFreqLds ? 2 : 1;
+  const int lane = threadIdx.x;
+  const GridView gv = P.gv;
+  const int span = P.span;
+  for (int chunk = l0; chunk < l1; chunk += 64 * RL) {
+    const int first = chunk + RL * lane;
+    const bool act = first < l1;
+    CoreLine L[RL];
+    int cb = 0x7fffffff - 64 + lane;            // lanes without lines: singleton segments out of reach
+    int lo = 0x3fffffff, hi = -0x3fffffff;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int l = first + r;
+      L[r].nu = 1e-6;
+      L[r].g = 0.0;
+      L[r].y = 15.0;
+      L[r].amp = 0.0f;
+      L[r].lo = 0x3fffffff;
+      L[r].hi = -0x3fffffff;
+      if (l < l1) {
+        const double v0 = P.nu[l];
+        L[r].nu = v0 == 0.0 ? 1e-6 : v0;
+        if (r == 0) cb = centre_bin(L[r].nu, gv);
+        L[r].g = tk.g[l];
+        L[r].y = tk.y[l];
+        L[r].amp = tk.amp[l];
+        const int2 core = tk.core[l];
+        if (core.y > core.x) {
+          L[r].lo = max(core.x, base) - cb;
+          L[r].hi = min(core.y, base + span) - cb;
+        }
+        lo = min(lo, L[r].lo);
+        hi = max(hi, L[r].hi);
+      }
+    }
+    if (!__ballot(hi > lo)) continue;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) L[r].r2 = W4Region2(L[r].y);
+    const int kmin = __builtin_amdgcn_readfirstlane(wave_min(lo));
+    const int kend = __builtin_amdgcn_readfirstlane(wave_max(hi));
+    // segments: lanes with equal base bins (contiguous, nu is sorted); each lane's distance
+    // from its segment's first lane fixes which steps of the scan it takes
+    const int cprev = __shfl_up(cb, 1), cnext = __shfl_down(cb, 1);
+    const bool head = lane == 0 || cprev != cb;
+    const bool tail = lane == 63 || cnext != cb;
+    const unsigned long long upto = __ballot(head) & ((2ull << lane) - 1ull);
+    const int hd = lane - (63 - __builtin_clzll(upto));
+    const float m1 = hd >= 1 ? 1.0f : 0.0f, m2 = hd >= 2 ? 1.0f : 0.0f, m4 = hd >= 4 ? 1.0f : 0.0f;
+    const float m8 = hd >= 8 ? 1.0f : 0.0f;
+    const float m15 = hd >= (lane & 15) + 1 ? 1.0f : 0.0f;   // lane 16 row - 1 is in my segment
+    const float m31 = hd >= lane - 31 ? 1.0f : 0.0f;          // lane 31 is in my segment
+    // the lane's row entries for offset 0 (a segment's lanes share them; lanes without
+    // lines never use theirs)
+    const int rel = act ? cb - base : 0;
+    double* const acc0 = s_core + kStride * rel;
+    const double* const fq0 = kFreqLds ? acc0 + 1 : gv.freq + (base + rel);
+    for (int k = kmin; k < kend; ++k) {
+      float v = 0.0f;
+      if (k >= lo && k < hi) {
+        const double f = fq0[kStride * k];
+#pragma unroll
+        for (int r = 0; r < RL; ++r)
+          if (k >= L[r].lo && k < L[r].hi) {
+            const double x = fabs(L[r].nu - f) * L[r].g;
+            v = __builtin_fmaf(L[r].amp, w4_core_mixed(x, L[r].y, L[r].r2), v);
+          }
+      }
+      v = seg_scan(v, m1, m2, m4, m8, m15, m31);
+      // a nonzero total has a line of the segment on at k, so entry cb + k - base is in the row
+      if (tail && v != 0.0f) atomicAdd(acc0 + kStride * k, (double)v);   // one lane per address
+    }
+  }
+}
+
+// At most 128 VGPRs (four waves per SIMD; with 4 lines per lane the compiler would take 136
+// and run three): the offset loop waits on the scan's DPP steps and the LDS, so the fourth
+// wave pays (176 ms against 191 ms per 48-point table); tighter limits spill.
 template <bool kFreqLds>
-__global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_core_mixed(MixedPlan P) {
   extern __shared__ double s_core[];
   constexpr int kStride = kFreqLds ? 2 : 1;
   const int lane = threadIdx.x;
@@ -941,57 +1032,10 @@ __global__ __launch_bounds__(64) void k_core_mixed(MixedPlan P) {
   __syncthreads();
   const int l0 = P.starts[w * P.cpw];
   const int l1 = P.starts[min((w + 1) * P.cpw, P.ncent)];
-  for (int chunk = l0; chunk < l1; chunk += 64) {
-    const int l = chunk + lane;
-    const bool act = l < l1;
-    double nu = 1e-6, g = 0.0, y = 15.0;
-    float amp = 0.0f;
-    int2 core = make_int2(0, 0);
-    int cj = 0x7fffffff - 64 + lane;            // inactive lanes: singleton segments out of reach
-    if (act) {
-      const double v0 = P.nu[l];
-      nu = v0 == 0.0 ? 1e-6 : v0;
-      g = tk.g[l];
-      y = tk.y[l];
-      amp = tk.amp[l];
-      core = tk.core[l];
-      cj = centre_bin(nu, gv);
-    }
-    const bool has = act && core.y > core.x;
-    if (!__ballot(has)) continue;
-    const W4Region2 r2(y);
-    // offsets k (bin cj + k) of this lane's core inside the row: [lo, hi)
-    const int lo = has ? max(core.x, base) - cj : 0x3fffffff;
-    const int hi = has ? min(core.y, base + span) - cj : -0x3fffffff;
-    const int kmin = __builtin_amdgcn_readfirstlane(max(wave_min(lo), -P.kmax));
-    const int kend = __builtin_amdgcn_readfirstlane(min(wave_max(hi), P.kmax + 1));
-    // segments: lanes with equal centre bins (contiguous, nu is sorted); each lane's distance
-    // from its segment's first lane fixes which steps of the scan it takes
-    const int cprev = __shfl_up(cj, 1), cnext = __shfl_down(cj, 1);
-    const bool head = lane == 0 || cprev != cj;
-    const bool tail = lane == 63 || cnext != cj;
-    const unsigned long long upto = __ballot(head) & ((2ull << lane) - 1ull);
-    const int hd = lane - (63 - __builtin_clzll(upto));
-    const float m1 = hd >= 1 ? 1.0f : 0.0f, m2 = hd >= 2 ? 1.0f : 0.0f, m4 = hd >= 4 ? 1.0f : 0.0f;
-    const float m8 = hd >= 8 ? 1.0f : 0.0f;
-    const float m15 = hd >= (lane & 15) + 1 ? 1.0f : 0.0f;   // lane 16 row - 1 is in my segment
-    const float m31 = hd >= lane - 31 ? 1.0f : 0.0f;          // lane 31 is in my segment
-    // the lane's row entries for offset 0 (a segment's lanes share them; inactive lanes
-    // never use theirs)
-    const int rel = act ? cj - base : 0;
-    double* const acc0 = s_core + kStride * rel;
-    const double* const fq0 = kFreqLds ? acc0 + 1 : gv.freq + (base + rel);
-    for (int k = kmin; k < kend; ++k) {
-      float v = 0.0f;
-      if (k >= lo && k < hi) {
-        const double x = fabs(nu - fq0[kStride * k]) * g;
-        v = amp * w4_core_mixed(x, y, r2);
-      }
-      v = seg_scan(v, m1, m2, m4, m8, m15, m31);
-      // a nonzero total has a lane of the segment on at k, so entry cj + k - base is in the row
-      if (tail && v != 0.0f) atomicAdd(acc0 + kStride * k, (double)v);   // one lane per address
-    }
-  }
+  if (l1 - l0 >= kCoreDenseLines * P.cpw)
+    core_chunks<kFreqLds, kCoreLaneLines>(P, tk, s_core, base, l0, l1);
+  else
+    core_chunks<kFreqLds, 1>(P, tk, s_core, base, l0, l1);
   __syncthreads();
   double* row = P.core_part + ((size_t)blockIdx.y * P.ncwg + w) * span;
   for (int i = lane; i < span; i += 64) row[i] = s_core[kStride * i];
