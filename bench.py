@@ -595,6 +595,10 @@ def main():
     freq, d = make_grid(start, end, M)
     gam = ll.gamma_at(P)
 
+    # the workload is one launch over the whole list: GetNtrans() (2^26 by default, the chunk
+    # the reference's driver would cut the list into) raised to hold it
+    os.environ["GX_MAX_NTRANS"] = str(max(1 << 26, 1 << int(np.ceil(np.log2(ll.n)))))
+
     def run(precision, steps, warmup):
         m = Mi355xManager(capi.VOIGT, precision=precision, cutoff=wl["cutoff"])
         m.InitializeVectors(M)

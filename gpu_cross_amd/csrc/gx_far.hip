@@ -925,8 +925,15 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   __shared__ int s_cut[NB];
   __shared__ double s_tot[R * RangeMoments<CPLX>::N];
   __shared__ double s_red[R];
-  const int b = blockIdx.x;
-  const int task = blockIdx.y;
+  // (block, point) pairs in block-major order, dealt to the XCDs (workgroups go to them in
+  // turn) in contiguous runs: an XCD takes the points of one block one after another, so the
+  // block's window-end lines (nu for every point, 1/alpha and amp for each temperature) come
+  // from its L2 instead of HBM; the launch pads gridDim.x to a multiple of 8
+  const int per = gridDim.x / 8;
+  const int pair = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (pair >= P.nblocks * P.ntasks) return;
+  const int b = pair / P.ntasks;
+  const int task = pair - b * P.ntasks;
   const size_t slot = (size_t)task * P.nblocks + b;
   const int4 er = P.edge_rng[slot];
   if (er.x >= er.y && er.z >= er.w) return;                    // set for whole blocks only
@@ -980,7 +987,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
 hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
-  const dim3 grid_blocks(plan.nblocks, plan.ntasks);
+  const dim3 grid_blocks(((unsigned)plan.nblocks * (unsigned)plan.ntasks + 7u) / 8u * 8u);   // k_far_edges
   switch (R) {
     case 1:
       k_far_eval<1><<<grid, 256, 0, s>>>(plan);
