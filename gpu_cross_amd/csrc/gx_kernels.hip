@@ -539,11 +539,13 @@ struct MixedShape {
 // Staged line record, 2 + ceil(R/4) float4 per line:
 //   [0]      ib, ie, k0, k1 (ints: window and core ranges, absolute bins); anchored fast
 //            lines keep x at the middles of slices 0-3 (those of the R slices) here instead
-//            (they need no ranges)
+//            (they need no ranges); edge lines with R <= 2 keep the 64-bit lane mask of each
+//            slice (bins inside the window and outside the core) as two ints per slice
 //   [1]      e, h, k, a      e = dnu/alpha; h = y^2 - 1/2, k = 2 y^2, a = amp c y (scaled)
 //   [2]      b, r, cf, s     b = a (scaled); r = x at the centre bin; cf: centre bin
 //                            relative to the wave's first bin; s: slice codes of an edge
-//                            line (int bits)
+//                            line (int bits); anchored edge lines (R <= 2) keep x at the
+//                            middles of slices 0 and 1 in r and cf
 //   [3 ...]  anchored fast lines: x at the middles of slices 4-7 (R = 8)
 // so the fast loop reads everything of a line from one LDS address.
 template <int R>
@@ -563,6 +565,15 @@ constexpr double kAnchorMaxE = 8.0;
 constexpr int kSliceMiss = 0;   // the window misses the slice
 constexpr int kSliceFull = 1;   // the window covers the slice and the core misses it
 constexpr int kSlicePart = 2;   // per-lane window and core test
+
+// Bits [a, b) of a 64-bit lane mask, a and b clamped to [0, 64].
+__device__ __forceinline__ unsigned long long lane_range(int a, int b) {
+  a = min(max(a, 0), 64);
+  b = min(max(b, 0), 64);
+  if (b <= a) return 0ull;
+  const unsigned long long below_b = b >= 64 ? ~0ull : (1ull << b) - 1ull;
+  return below_b & ~((1ull << a) - 1ull);     // a < 64 here
+}
 
 // Kahan summation with the compensation kept in the running partial sum: part holds
 // the terms added since the last step plus the rounding error of that step (Fast2Sum
@@ -670,7 +681,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
       if (base >= er.z && base < er.w) base = er.w;
       if (base >= hi) break;
       const int i = base + lane;
-      bool fa = false, fx = false, ed = false;
+      bool fa = false, fx = false, ea = false, ed = false;
       float4 rec[kVec];
       if (i < hi && !(i >= fr.x && i < fr.y) && !(i >= fr.z && i < fr.w) && !(i >= er.x && i < er.y) &&
           !(i >= er.z && i < er.w)) {
@@ -703,6 +714,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
         fa = full && !cor && an;
         fx = full && !cor && !an;
         ed = touch && !(full && !cor);
+        ea = R <= 2 && ed && an;                   // edge lines with slice-middle anchors
+        ed = ed && !ea;
         if (fa) {
           // slice q of the wave has its middle at bin w0 + 64 q + 32
           const double ed_ = gv.dfreq * g * sg;
@@ -714,8 +727,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
                                                     (float)(b0 - 128.0 * ed_), (float)(b0 - 192.0 * ed_));
           }
         } else {
-          rec[0] = make_float4(__int_as_float(ib), __int_as_float(ie), __int_as_float(core.x),
-                               __int_as_float(core.y));
           // slice codes of an edge line, 2 bits per 64-bin slice
           int codes = 0;
 #pragma unroll
@@ -728,18 +739,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
                               : kSlicePart;
             codes |= c << (2 * q);
           }
+          if constexpr (R <= 2) {
+            // per slice, the lanes inside the window and outside the core, as a lane mask
+            unsigned long long mk[2] = {0ull, 0ull};
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+              const int s0 = w0 + 64 * q;
+              const unsigned long long cm = core.y > core.x ? lane_range(core.x - s0, core.y - s0) : 0ull;
+              mk[q] = lane_range(ib - s0, ie - s0) & ~cm;
+            }
+            rec[0] = make_float4(__uint_as_float((unsigned)mk[0]), __uint_as_float((unsigned)(mk[0] >> 32)),
+                                 __uint_as_float((unsigned)mk[1]), __uint_as_float((unsigned)(mk[1] >> 32)));
+            if (ea) {
+              // x at the middles of slices 0 and 1, as the anchored fast lines keep them
+              const double ed_ = gv.dfreq * g * sg;
+              const double x32 = (nu - fc) * g * sg + (cj - (double)(w0 + 32)) * ed_;
+              rec[2].y = (float)x32;
+              rec[2].z = (float)(x32 - 64.0 * ed_);
+            }
+          } else {
+            rec[0] = make_float4(__int_as_float(ib), __int_as_float(ie), __int_as_float(core.x),
+                                 __int_as_float(core.y));
+          }
           rec[2].w = __int_as_float(codes);
         }
       }
       // Compact the chunk's records by class, in line order: anchored fast lines first, then
-      // exact-offset fast lines, then edge lines; lines that miss the wave are dropped.
-      const unsigned long long ma = __ballot(fa), mx = __ballot(fx), me = __ballot(ed);
-      const int na = __popcll(ma), nx = __popcll(mx), ne = __popcll(me);
-      const unsigned long long mine = fa ? ma : (fx ? mx : me);
-      const int pos = (fx ? na : 0) + (ed ? na + nx : 0) + __builtin_amdgcn_mbcnt_hi(
-          (unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+      // exact-offset fast lines, anchored edge lines (R <= 2), then the other edge lines;
+      // lines that miss the wave are dropped.
+      const unsigned long long ma = __ballot(fa), mx = __ballot(fx), mea = __ballot(ea), me = __ballot(ed);
+      const int na = __popcll(ma), nx = __popcll(mx), nea = __popcll(mea), ne = __popcll(me);
+      const unsigned long long mine = fa ? ma : (fx ? mx : (ea ? mea : me));
+      const int pos = (fx ? na : 0) + (ea ? na + nx : 0) + (ed ? na + nx + nea : 0) +
+                      __builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
       wave_lds_handover();                         // the previous chunk's reads are done
-      if (fa || fx || ed) {
+      if (fa || fx || ea || ed) {
         my_l[pos][0] = rec[0];
         my_l[pos][1] = rec[1];
         my_l[pos][2] = rec[2];
@@ -777,31 +811,76 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
       });
       // edge lines: per 64-bin slice the staged slice code says whether the line misses it
       // (skipped), covers it clear of its core (no per-lane test) or covers part of it (the
-      // per-lane window and core test); the core bins [k0, k1) belong to the core pass
-      run_block(ne, [&](int a, int z) {
-        for (int k = na + nx + a; k < na + nx + z; ++k) {
-          const float4 Iv = my_l[k][0];
-          const float4 E = my_l[k][1];
-          const float4 B = my_l[k][2];
-          const int codes = __builtin_amdgcn_readfirstlane(__float_as_int(B.w));
-#pragma unroll
-          for (int q = 0; q < R; ++q) {
-            const int c = (codes >> (2 * q)) & 3;
-            if (c == kSliceMiss) continue;
-            const float x = __builtin_fmaf(B.z - lanef_q[q], E.x, B.y);
-            const float u = __builtin_fmaf(x, x, E.y);
-            const float rden = __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, E.z));
-            float num = __builtin_fmaf(E.w, u, B.x);
-            if (c == kSlicePart) {
-              const int jj = w0 + lane + 64 * q;
-              const bool ok = jj >= __float_as_int(Iv.x) && jj < __float_as_int(Iv.y) &&
-                              (jj < __float_as_int(Iv.z) || jj >= __float_as_int(Iv.w));
-              num = ok ? num : 0.0f;
-            }
-            part[q] = __builtin_fmaf(num, rden, part[q]);
+      // per-lane window and core test); the core bins [k0, k1) belong to the core pass.  With
+      // R <= 2 the test is the staged lane mask (one select), and the lines with
+      // e <= kAnchorMaxE, a class of their own, form x from the slice-middle anchors (one FMA,
+      // as the anchored fast lines do).
+      auto edge_slice = [&](int q, int c, float x, const float4& E, const float4& B, const float4& Iv) {
+        const float u = __builtin_fmaf(x, x, E.y);
+        const float rden = __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, E.z));
+        float num = __builtin_fmaf(E.w, u, B.x);
+        if (c == kSlicePart) {
+          if constexpr (R <= 2) {
+            const unsigned lo = __builtin_amdgcn_readfirstlane(__float_as_uint(q == 0 ? Iv.x : Iv.z));
+            const unsigned hi = __builtin_amdgcn_readfirstlane(__float_as_uint(q == 0 ? Iv.y : Iv.w));
+            num = __builtin_amdgcn_inverse_ballot_w64(((unsigned long long)hi << 32) | lo) ? num : 0.0f;
+          } else {
+            const int jj = w0 + lane + 64 * q;
+            const bool ok = jj >= __float_as_int(Iv.x) && jj < __float_as_int(Iv.y) &&
+                            (jj < __float_as_int(Iv.z) || jj >= __float_as_int(Iv.w));
+            num = ok ? num : 0.0f;
           }
         }
-      });
+        part[q] = __builtin_fmaf(num, rden, part[q]);
+      };
+      if constexpr (R <= 2) {
+        // the lane mask is zero on a slice the line misses and all ones on one it covers clear
+        // of its core, so the loops take every slice without a branch and two lines at a time
+        auto masked = [&](int q, float x, const float4& E, const float4& B, const float4& Iv) {
+          const float u = __builtin_fmaf(x, x, E.y);
+          const float rden = __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, E.z));
+          const unsigned lo = __builtin_amdgcn_readfirstlane(__float_as_uint(q == 0 ? Iv.x : Iv.z));
+          const unsigned hi = __builtin_amdgcn_readfirstlane(__float_as_uint(q == 0 ? Iv.y : Iv.w));
+          const float num = __builtin_amdgcn_inverse_ballot_w64(((unsigned long long)hi << 32) | lo)
+                                ? __builtin_fmaf(E.w, u, B.x)
+                                : 0.0f;
+          part[q] = __builtin_fmaf(num, rden, part[q]);
+        };
+        run_block(nea, [&](int a, int z) {
+#pragma unroll 2
+          for (int k = na + nx + a; k < na + nx + z; ++k) {
+            const float4 Iv = my_l[k][0];
+            const float4 E = my_l[k][1];
+            const float4 B = my_l[k][2];
+#pragma unroll
+            for (int q = 0; q < R; ++q) masked(q, __builtin_fmaf(no, E.x, q == 0 ? B.y : B.z), E, B, Iv);
+          }
+        });
+        run_block(ne, [&](int a, int z) {
+#pragma unroll 2
+          for (int k = na + nx + nea + a; k < na + nx + nea + z; ++k) {
+            const float4 Iv = my_l[k][0];
+            const float4 E = my_l[k][1];
+            const float4 B = my_l[k][2];
+#pragma unroll
+            for (int q = 0; q < R; ++q) masked(q, __builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E, B, Iv);
+          }
+        });
+      } else {
+        run_block(ne, [&](int a, int z) {
+          for (int k = na + nx + nea + a; k < na + nx + nea + z; ++k) {
+            const float4 Iv = my_l[k][0];
+            const float4 E = my_l[k][1];
+            const float4 B = my_l[k][2];
+            const int codes = __builtin_amdgcn_readfirstlane(__float_as_int(B.w));
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+              const int c = (codes >> (2 * q)) & 3;
+              if (c != kSliceMiss) edge_slice(q, c, __builtin_fmaf(B.z - lanef_q[q], E.x, B.y), E, B, Iv);
+            }
+          }
+        });
+      }
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) kahan_step(sum[q], part[q]);
