@@ -29,7 +29,12 @@ sys.path.insert(0, ROOT)
 FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compensated add
 # HBM bytes per accumulation launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # same command scaled by a plain-copy calibration (profiles/k_accum_voigt_mixed.table_48.hbm.md)
-PROFILED_TRAFFIC = {"table_48": 2.903e11}
+# From the rocprofv3 summaries under profiles/, per workload and block size (64 R bins; the
+# table's autotuner picks R): HBM bytes per launch of the region-I kernel (2 FETCH_SIZE +
+# WRITE_SIZE, copy-calibrated), and the step's VALU issue use, sum over every kernel of one
+# 48-point table of SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x clock x step time).
+PROFILED_TRAFFIC = {("table_48", 1): 3.750e11, ("table_48", 2): 2.903e11, "ingest_1e8": None}
+PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.887, "step_ms": 494.4, "valu_insts": 2.60e11, "clock_ghz": 2.317}}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -200,7 +205,7 @@ def bench_table(args, wl):
         "roofline": {"bound": "valu", "achieved": FLOP_PER_EVAL * r1_evals / (r1_ms * 1e-3) / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": FLOP_PER_EVAL * r1_evals / (r1_ms * 1e-3) / PEAK_FP32,
-                     "traffic": PROFILED_TRAFFIC.get("table_48"),
+                     "traffic": PROFILED_TRAFFIC.get(("table_48", tab.shape()[0])),
                      "traffic_source": "rocprofv3 2*FETCH_SIZE+WRITE_SIZE per launch (copy-calibrated), profiles/",
                      "kernel": "k_accum_voigt_mixed", "kernel_ms": r1_ms, "evals": r1_evals,
                      "flop_per_eval": FLOP_PER_EVAL},
@@ -209,6 +214,10 @@ def bench_table(args, wl):
         "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, k_far_edges (FP32 leaf sums, FP64 tree and series)",
                       "ms": far_ms, "evals": far_evals, "share": far_evals / max(evals, 1)},
         "intensity_kernels_ms": eager_int,
+        "step_valu_issue": dict(PROFILED_STEP_VALU.get(("table_48", tab.shape()[0]), {}),
+                                source="rocprofv3 --pmc SQ_INSTS_VALU GRBM_GUI_ACTIVE over one 48-point table "
+                                       "(profiles/), at the block size this run picked")
+        if ("table_48", tab.shape()[0]) in PROFILED_STEP_VALU else None,
     }
     mixed_grids = [tab.read(p) for p in range(len(pts))]
     tab.close()

@@ -98,7 +98,11 @@ struct MixedRunner {
   double* core_part = nullptr; // core pass: per-workgroup bin rows
   size_t core_cap = 0;
   int cus = 0;
-  int R = 4;                   // bins per lane (1, 2, 4 or 8): blocks of 64 R bins
+  // bins per lane (1, 2, 4 or 8): blocks of 64 R bins.  The manager and the server take 2: the
+  // far field then takes most lines of a block and the branch-free masked edge loop (R <= 2)
+  // the rest (one 1e8-line cross-section: 27.9 ms at R = 2, 27.7 at 1, 31.3 at 4); tables
+  // autotune it
+  int R = 2;
   int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
   int forced_nseg = 0;         // > 0: fixed segments per block
   bool persistent = true;      // false: one workgroup per work item, so the hardware interleaves
