@@ -129,6 +129,12 @@ def voigt916(x, y, a: float = 1.0) -> np.ndarray:
     return out
 
 
+def gh3(x: float, y: float) -> float:
+    """The 3-point Gauss-Hermite expression of the Voigt sum without per-line gamma
+    (cross_kernal.cu:183-186), at one (x, y)."""
+    return lib().orc_gh3(float(x), float(y), float(x) * float(x) + float(y) * float(y))
+
+
 def erfcx(x) -> np.ndarray:
     x = np.ascontiguousarray(np.asarray(x, np.float64)).ravel()
     out = np.empty_like(x)

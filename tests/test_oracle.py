@@ -314,6 +314,26 @@ def test_gaussian_telescoping_sum_and_half_width():
     assert po.count_evals(c, np.array([nu]), freq, freq.size, 0) == inside.size
 
 
+def test_gh3_is_a_three_node_gauss_hermite_sum():
+    """The 3-point expression is a sum over the Gauss-Hermite nodes 0 and +-sqrt(3/2) of
+    Lorentzians: w0 y / (pi (x^2 + y^2)) + w1 (1/((x - a)^2 + y^2) + 1/((x + a)^2 + y^2)),
+    a^2 = 3/2, w1 = sqrt(pi)/6 (the rule's outer weight), with the reference's centre weight
+    1.1181635900 (the rule's 2 sqrt(pi)/3 = 1.1816359006 with two digits swapped) and no
+    y/pi on the outer pair, both as the reference writes them.  Summing the nodes one by one
+    checks the oracle's combined rational form (the pair's common denominator
+    (x^2 + y^2 + a^2)^2 - 4 a^2 x^2)."""
+    a2 = 1.5
+    w0, w1 = 1.1181635900, 0.2954089751                 # the literals (w1 = sqrt(pi)/6 to 10 digits)
+    assert abs(w1 - math.sqrt(math.pi) / 6.0) < 1e-10 and abs(2.0 * math.sqrt(math.pi) / 3.0 - 1.1816359006) < 1e-10
+    rng = np.random.default_rng(11)
+    for x, y in zip(rng.uniform(-30.0, 30.0, 400), 10 ** rng.uniform(-3.0, 1.5, 400)):
+        if x * x + y * y < 100.0:
+            continue                                  # the sum uses the 3-point form from 100 up
+        want = (w0 * y / (3.14159265359 * (x * x + y * y)) +
+                w1 * (1.0 / ((x - math.sqrt(a2)) ** 2 + y * y) + 1.0 / ((x + math.sqrt(a2)) ** 2 + y * y)))
+        assert po.gh3(x, y) == pytest.approx(want, rel=1e-12)
+
+
 def test_nogamma_voigt_lorentz_limit_with_mass_in_amu():
     """With the mean mass in amu the no-gamma sum's Doppler width (no Avogadro factor) is
     ~1e-16 cm^-1, every bin takes the 3-point form, and the profile reduces (to O(gG/gamma_L)) to
