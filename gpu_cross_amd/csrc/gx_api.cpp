@@ -338,6 +338,7 @@ int MixedRunner::init() {
     if (r == 1 || r == 2 || r == 4 || r == 8) R = r;
   }
   if (const char* env = std::getenv("GX_FAR")) far_on = std::atoi(env) != 0;
+  if (const char* env = std::getenv("GX_NEAR")) near_on = std::atoi(env) != 0;
   hipError_t e = hipMalloc(&counter, sizeof(unsigned));
   if (e) return hip_fail(e, "hipMalloc(counter)");
   if ((e = hipMalloc(&far_evals, sizeof(unsigned long long))) || (e = hipEventCreate(&far_ev[0])) ||
@@ -359,6 +360,8 @@ void MixedRunner::release() {
   far_geom = nullptr; far_rec = nullptr; far_rng = nullptr; edge_rng = nullptr; far_list = nullptr; far_cnt = nullptr;
   far_evals = nullptr; far_kind = nullptr; far_geom_cap = far_rec_cap = far_blk_cap = 0; far_kind_cap = 0;
   far_first = nullptr; far_first_cap = 0;
+  hipFree(near_u); hipFree(near_rng);
+  near_u = nullptr; near_rng = nullptr; near_u_cap = near_rng_cap = 0;
   for (auto& ev : far_ev) {
     if (ev) hipEventDestroy(ev);
     ev = nullptr;
@@ -371,7 +374,7 @@ size_t MixedRunner::bytes() const {
   return (size_t)task_cap * sizeof(MixedTask) + partial_cap * sizeof(float2) + starts_cap * sizeof(int) +
          core_cap * sizeof(double) + far_geom_cap * sizeof(double4) + far_rec_cap * sizeof(double) +
          far_blk_cap * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + (size_t)far_kind_cap * sizeof(int) +
-         2 * sizeof(int) * (size_t)far_first_cap;
+         2 * sizeof(int) * (size_t)far_first_cap + near_u_cap * sizeof(int2) + near_rng_cap * sizeof(int4);
 }
 
 int MixedRunner::far_time(float* ms, unsigned long long* evals) {
@@ -428,7 +431,7 @@ CoreGeom core_geometry(const GridView& gv, int kmax) {
 size_t core_rows_bytes(const GridView& gv, double dpw) {
   const int kmax = core_kmax(gv, dpw);
   if (kmax < 0 || gv.points <= 0) return 0;
-  const CoreGeom g = core_geometry(gv, kmax);
+  const CoreGeom g = core_geometry(gv, kmax + kNearMargin);   // room for near ranges
   return (size_t)g.ncwg * (size_t)g.span * sizeof(double);
 }
 
@@ -487,6 +490,8 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
   plan.queue = persistent ? 1 : 0;
 
   plan.nlines = std::max(nlines, 0);
+  const bool far_plan = far_on && plan.nlines >= 2 * kFarLeaf && gv.points > 0;
+  bool near_plan = far_plan && near_on;
   if (gv.points > 0 && plan.nlines > 0) {
     int kmax = 0;
     for (int k = 0; k < ntasks; ++k) {
@@ -494,6 +499,9 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
       if (kk < 0) return fail(GX_INVALID_ARG, "mixed: W4 core too wide for the core pass; use FP64 for this grid");
       kmax = std::max(kmax, kk);
     }
+    // near ranges reach a little beyond the cores: kNearMargin more bins of row where LDS holds them
+    if (near_plan && core_lds_bytes(64 + 2 * (kmax + kNearMargin) + 1)) kmax += kNearMargin;
+    else near_plan = false;
     const CoreGeom cg = core_geometry(gv, kmax);
     plan.kmax = kmax;
     plan.cpw = cg.cpw;
@@ -526,7 +534,9 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
   }
   // far field: leaves of kFarLeaf lines and their binary tree
   plan.far = 0;
-  if (far_on && plan.nlines >= 2 * kFarLeaf && gv.points > 0) {
+  plan.near = 0;
+  plan.core_evals = core_counter;
+  if (far_plan) {
     int n = (plan.nlines + kFarLeaf - 1) / kFarLeaf, lev = 0, tot = 0;
     while (true) {
       if (lev >= kFarMaxLevels) return fail(GX_INVALID_ARG, "mixed: too many far-field levels");
@@ -604,6 +614,29 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
     plan.far_ib1 = far_first + far_first_cap;
     plan.far_bin_cap = far_first_cap;
     plan.far_evals = far_evals;
+    if (near_plan) {
+      const size_t need_u = (size_t)ntasks * tot;
+      if (need_u > near_u_cap || need_blk > near_rng_cap) {
+        if ((e = hipStreamSynchronize(s))) return hip_fail(e, "mixed: sync");
+        if (need_u > near_u_cap) {
+          hipFree(near_u);
+          near_u = nullptr;
+          near_u_cap = 0;
+          if ((e = hipMalloc(&near_u, sizeof(int2) * need_u))) return hip_fail(e, "hipMalloc(near ranges)");
+          near_u_cap = need_u;
+        }
+        if (need_blk > near_rng_cap) {
+          hipFree(near_rng);
+          near_rng = nullptr;
+          near_rng_cap = 0;
+          if ((e = hipMalloc(&near_rng, sizeof(int4) * need_blk))) return hip_fail(e, "hipMalloc(near runs)");
+          near_rng_cap = need_blk;
+        }
+      }
+      plan.near = 1;
+      plan.near_u = near_u;
+      plan.near_rng = near_rng;
+    }
   }
   return GX_OK;
 }
@@ -611,8 +644,8 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
 // Bytes of the far-field buffers for ntasks points over nlines lines and nblocks blocks.
 size_t far_field_bytes(int ntasks, long long nlines, int nblocks) {
   const long long clusters = 2 * ((nlines + kFarLeaf - 1) / kFarLeaf) + 64;
-  return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * kFarRec * sizeof(double)) +
-         (size_t)ntasks * nblocks * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + sizeof(int) * (size_t)ntasks +
+  return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * (kFarRec * sizeof(double) + sizeof(int2))) +
+         (size_t)ntasks * nblocks * (3 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + sizeof(int) * (size_t)ntasks +
          2 * sizeof(int) * ((size_t)nblocks * 64 + 1);    // per-bin line bounds (callers count 64-bin blocks)
 }
 
@@ -659,6 +692,7 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
     if ((rc = der.reserve_mixed((size_t)std::max(nlines, 1), 1, 1))) return rc;
     if ((rc = der.set_points(&pc, 1, s))) return rc;
     MixedTask task = mixed_task(der.m, 0, 0, pc, intens);
+    mix.core_counter = d_core_evals;
     if ((rc = mix.prepare(&task, 1, nblocks, gv, nlines, s))) return rc;
   }
   if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");

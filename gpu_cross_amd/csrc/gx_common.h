@@ -123,6 +123,13 @@ struct MixedRunner {
   int* far_first = nullptr;    // first line above each bin: ie (entries [0, cap)), then ib
   int far_first_cap = 0;       // entries per array
   unsigned long long* far_evals = nullptr;
+  // near leaves (real-centred points; GX_NEAR=0 switches them off)
+  bool near_on = true;
+  int2* near_u = nullptr;
+  size_t near_u_cap = 0;       // (task, cluster) entries
+  int4* near_rng = nullptr;
+  size_t near_rng_cap = 0;     // (task, block) slots
+  unsigned long long* core_counter = nullptr;   // optional kEvalSlots counters of core-pass evaluations
   hipEvent_t far_ev[2] = {nullptr, nullptr};
   bool far_timed = false;      // the last launch recorded far_ev
   std::vector<MixedTask> h_tasks;

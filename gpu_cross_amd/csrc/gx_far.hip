@@ -246,6 +246,36 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
   }
 }
 
+// Near range U of a leaf at a real-centred point: the bins left to the core pass for all its
+// lines, namely every bin of a line's W4 core (|nu - f| g + y < 15 implies |nu - f| < (15 - ymin)
+// amax, with the leaf's outward-rounded extremes) and every bin nearer to the leaf's centre than
+// r / kNearRho, where its series would not meet the truncation bound.  Elsewhere in the blocks
+// whose near lines it holds, every pair of the leaf is a region-I pair and the series converges.
+// Empty (not near-eligible) unless the leaf is full, U lies within kmax of every line's centre bin
+// (the core pass rows hold it) and inside every line's window (ib, ie rise with nu).
+__device__ __forceinline__ int2 near_range(const MixedPlan& P, const MixedTask& tk, int leaf, double r, double amax,
+                                           double ymin) {
+#pragma clang fp contract(off)
+  const int l0 = leaf * kFarLeaf, l1 = l0 + kFarLeaf - 1;
+  if (l1 >= P.nlines) return make_int2(0, 0);
+  const GridView gv = P.gv;
+  const double4 gm = P.far_geom[leaf];                       // centre, lowest nu, highest nu
+  // x = |nu - f| / alpha stays below 1e6 over the core pass rows (FP32 region I there)
+  if (!(gm.y * tk.dpwcoef * 1e6 >= (double)(P.kmax + 2) * gv.dfreq)) return make_int2(0, 0);
+  const double h = ymin < 15.0 ? (15.0 - ymin) * amax : 0.0;
+  const double dc = r / kNearRho;
+  const double a = fmin(gm.y - h, gm.x - dc), z = fmax(gm.z + h, gm.x + dc);
+  // every bin j with a < f_j < z: j > (a - f_0) / dnu and j < (z - f_0) / dnu, with a margin of
+  // 1e-3 bin for the rounding of f_j and of these quotients
+  const double xa = floor((a - gv.start_nu) / gv.dfreq - 1e-3), xz = ceil((z - gv.start_nu) / gv.dfreq + 1e-3);
+  if (!(xa > -1e9 && xz < 1e9)) return make_int2(0, 0);
+  const int u0 = (int)xa, u1 = (int)xz;
+  const int cb0 = centre_bin(nu_if(P.nu[l0]), gv), cb1 = centre_bin(nu_if(P.nu[l1]), gv);
+  if (u0 < cb1 - P.kmax || u1 - 1 > cb0 + P.kmax) return make_int2(0, 0);
+  if (u0 < P.ib[l1] || u1 > P.ie[l0]) return make_int2(0, 0);
+  return make_int2(u0, u1);
+}
+
 // Leaf records of four consecutive leaves (one per wave) are staged in LDS and written field
 // by field, four doubles per field; workgroups are mapped so that each XCD takes a contiguous
 // run of leaves (workgroups go to the XCDs in turn), and the 128-byte lines of a field, shared
@@ -254,6 +284,7 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
 // write-backs from four L2s (54 ms against 61.6 ms per 48-point table).
 __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
   __shared__ double s_rec[4][kFarRec + 1];   // +1: the rows start in different banks
+  __shared__ double s_head[4][kLeafTasks][3];  // radius, max alpha, min y of each point (near ranges)
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int per = gridDim.x / 8;              // the launch pads gridDim.x to a multiple of 8
@@ -273,6 +304,12 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
     const MixedTask tk = P.tasks[task];
     const bool cplx = P.far_kind[task] != 0;
     if (valid) leaf_record(tk, cplx, l, act, nu, c, lane, FarRec<double>{&s_rec[wave][0], 1});
+    if (P.near && valid && lane == 0) {        // lane 0 wrote the head it copies
+      const int t = task - blockIdx.y * kLeafTasks;
+      s_head[wave][t][0] = s_rec[wave][0];
+      s_head[wave][t][1] = s_rec[wave][1];
+      s_head[wave][t][2] = cplx ? -1.0 : s_rec[wave][2];   // -1: complex centres, no near range
+    }
     __syncthreads();
     const int nf = cplx ? kFarRec : kFarHead + kFarP;
     double* dst = P.far_rec + (size_t)task * ns * kFarRec + leaf0;
@@ -281,6 +318,14 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
       if (lf < nleaf) dst[(size_t)f * ns + lf] = s_rec[lf][f];
     }
     __syncthreads();
+  }
+  // near ranges of the wave's leaf at its points, one lane per point
+  const int t = lane, task = blockIdx.y * kLeafTasks + lane;
+  if (P.near && valid && t < kLeafTasks && task < t1) {
+    const double ymin = s_head[wave][t][2];
+    P.near_u[(size_t)task * ns + leaf] =
+        ymin < 0.0 ? make_int2(0, 0)
+                   : near_range(P, P.tasks[task], leaf, s_head[wave][t][0], s_head[wave][t][1], ymin);
   }
 }
 
@@ -314,6 +359,15 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     r = fmax(r, (cplx ? sqrt(dx * dx + dy * dy) : fabs(dx)) + cr[0]);
   }
   if (!(r > 0.0)) r = 1.0;
+  if (P.near) {                                  // hull of the children's near ranges
+    int2 u = make_int2(0, 0);
+    for (int h = 0; h < nch; ++h) {
+      const int2 cu = P.near_u[(size_t)blockIdx.y * ns + P.far_off[lev - 1] + 2 * i + h];
+      if (cu.y <= cu.x) continue;
+      u = u.y > u.x ? make_int2(min(u.x, cu.x), max(u.y, cu.y)) : cu;
+    }
+    P.near_u[(size_t)blockIdx.y * ns + self] = u;
+  }
   const FarRec<double> rec{tb + self, ns};
   rec[0] = r;
   rec[1] = amax;
@@ -446,16 +500,23 @@ __device__ __forceinline__ bool series_ok(double r, double gamma_c, double gamma
               : (double)(P + 1) * rp * w <= kFarTol * (1.0 - rho) * (1.0 - rho);
 }
 
-// Whether cluster (geometry gm, record rc) serves every bin of [f_lo, f_hi]: no line of it
-// has its W4 core there (its lowest y is >= 15, or its lines are 15 Doppler widths away)
-// and its series converges there within the truncation bound of its kind.
+// Whether cluster (geometry gm, record rc, near range u) serves every bin of [f_lo, f_hi] =
+// bins [j_lo, j_hi]: no line of it has its W4 core there (its lowest y is >= 15, or its lines are
+// 15 Doppler widths away), no bin is in the near range of one of its leaves (the core pass has
+// those pairs), and its series converges there within the truncation bound of its kind.
 __device__ __forceinline__ bool serves(const double4 gm, FarRecC rc, bool cplx, double f_lo,
-                                       double f_hi) {
+                                       double f_hi, int2 u, int j_lo, int j_hi) {
 #pragma clang fp contract(off)
+  if (u.y > u.x && u.x <= j_hi && u.y > j_lo) return false;
   const double dist = fmax(fmax(f_lo - gm.x, gm.x - f_hi), 0.0);      // centre to the bins
   const double edge = fmax(fmax(f_lo - gm.z, gm.y - f_hi), 0.0);      // the lines' nu range to the bins
   if (!(rc[2] >= 15.0 || edge >= 15.001 * rc[1])) return false;
   return series_ok<kFarP>(rc[0], rc[5], rc[3], dist, cplx);
+}
+
+// Near range of cluster k of a task (empty when near leaves are off).
+__device__ __forceinline__ int2 near_u_of(const MixedPlan& P, int task, int k) {
+  return P.near ? P.near_u[(size_t)task * P.far_nclus + k] : make_int2(0, 0);
 }
 
 // Sum of a cluster (record rc, centre nu_c) at bin f.
@@ -490,7 +551,7 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
   if (b >= P.nblocks) return;
   const int task = blockIdx.y;
   const size_t slot = (size_t)task * P.nblocks + b;
-  int4 rng = make_int4(0, 0, 0, 0);
+  int4 rng = make_int4(0, 0, 0, 0), nrng = make_int4(0, 0, 0, 0);
   int cnt = 0;
   unsigned long long far_lines = 0;
   const GridView gv = P.gv;
@@ -508,7 +569,8 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
     const bool cplx = P.far_kind[task] != 0;
     auto usable = [&](int lev, int idx) -> bool {
       const int k = P.far_off[lev] + idx;
-      return serves(P.far_geom[k], FarRecC{recs + k, (size_t)P.far_nclus}, cplx, f_lo, f_hi);
+      return serves(P.far_geom[k], FarRecC{recs + k, (size_t)P.far_nclus}, cplx, f_lo, f_hi, near_u_of(P, task, k),
+                    w0, w1 - 1);
     };
     int i = leafL;
     if (leafL < leafR) {
@@ -542,10 +604,18 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
         }
         if (!took) break;
       }
-      rng = make_int4(leafL * kFarLeaf, i * kFarLeaf, j * kFarLeaf, leafR * kFarLeaf);
-      far_lines = (unsigned long long)(rng.y - rng.x) + (unsigned long long)(rng.w - rng.z);
+      far_lines = (unsigned long long)(i - leafL) * kFarLeaf + (unsigned long long)(leafR - j) * kFarLeaf;
+      // near leaves next to the far clusters on either side: summed per bin by k_near_eval
+      int i2 = i, j2 = j;
+      if (P.near) {
+        while (i2 < j && near_u_of(P, task, i2).y > near_u_of(P, task, i2).x) ++i2;
+        while (j2 > i2 && near_u_of(P, task, j2 - 1).y > near_u_of(P, task, j2 - 1).x) --j2;
+        nrng = make_int4(i, i2, j2, j);
+      }
+      rng = make_int4(leafL * kFarLeaf, i2 * kFarLeaf, j2 * kFarLeaf, leafR * kFarLeaf);
     }
   }
+  if (P.near) P.near_rng[slot] = nrng;
   P.far_rng[slot] = rng;
   P.far_cnt[slot] = cnt;
   if (far_lines) atomicAdd(P.far_evals, far_lines * (unsigned long long)P.far_bins);
@@ -584,7 +654,8 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
     auto leaves_usable = [&](int a, int z) -> bool {       // every lane calls it together
       bool ok = true;
       for (int leaf = a / kFarLeaf + lane; ok && leaf * kFarLeaf < z; leaf += 64)
-        ok = serves(P.far_geom[leaf], FarRecC{recs + leaf, (size_t)P.far_nclus}, cplx, f_lo, f_hi);
+        ok = serves(P.far_geom[leaf], FarRecC{recs + leaf, (size_t)P.far_nclus}, cplx, f_lo, f_hi,
+                    near_u_of(P, task, leaf), w0, w1 - 1);
       return __ballot(!ok) == 0ull;
     };
     if (em1 > em0 && leaves_usable(em0, em1)) {
@@ -659,6 +730,127 @@ __global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
   for (int q = 0; q < R; ++q) tk.out[w0 + lane + 64 * q] += acc[q];
 }
 
+// 1/d for d != 0: the hardware reciprocal refined by two Newton steps (the series are
+// approximations held to a bound far above this rounding).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+
+// Leaves staged per pass of k_near_eval.
+constexpr int kNearBatch = 32;
+
+// Blocks of 64 R bins per wave of k_near_eval: neighbouring blocks share most of their near
+// leaves, so one staging of a leaf's moments in LDS serves four bins per lane.
+template <int R>
+struct NearShape {
+  static constexpr int G = R >= 4 ? 1 : 4 / R;
+  static constexpr int Q = R * G;   // bins per lane
+};
+
+// Near leaves of G neighbouring (task, block) pairs per wave (the runs k_far_plan left between
+// the far clusters and each block's direct lines): bin j of block b adds a leaf of b's runs
+// wherever j is outside the leaf's near range (the core pass evaluates the leaf's lines at the
+// bins inside it).  The leaves of the blocks' runs are consecutive, so each field of
+// kNearBatch leaves is one coalesced load (field-major records), staged in LDS and read back
+// as broadcasts; a leaf outside a block's runs is skipped for that block (a scalar branch).
+template <int R>
+__global__ __launch_bounds__(256) void k_near_eval(MixedPlan P) {
+  constexpr int G = NearShape<R>::G, Q = NearShape<R>::Q;
+  __shared__ double s_q[4][kFarP][kNearBatch];
+  __shared__ double s_c[4][kNearBatch], s_r[4][kNearBatch];
+  __shared__ int2 s_u[4][kNearBatch];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b0 = (blockIdx.x * 4 + wave) * G;
+  if (b0 >= P.nblocks) return;
+  const int task = blockIdx.y;
+  int4 nr[G];
+  int Lmin = 0x7fffffff, Lmax = -1;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    nr[g] = b0 + g < P.nblocks ? P.near_rng[(size_t)task * P.nblocks + b0 + g] : make_int4(0, 0, 0, 0);
+    nr[g].x = __builtin_amdgcn_readfirstlane(nr[g].x);
+    nr[g].y = __builtin_amdgcn_readfirstlane(nr[g].y);
+    nr[g].z = __builtin_amdgcn_readfirstlane(nr[g].z);
+    nr[g].w = __builtin_amdgcn_readfirstlane(nr[g].w);
+    if (nr[g].y > nr[g].x) {
+      Lmin = min(Lmin, nr[g].x);
+      Lmax = max(Lmax, nr[g].y);
+    }
+    if (nr[g].w > nr[g].z) {
+      Lmin = min(Lmin, nr[g].z);
+      Lmax = max(Lmax, nr[g].w);
+    }
+  }
+  if (Lmin >= Lmax) return;
+  const MixedTask tk = P.tasks[task];
+  const GridView gv = P.gv;
+  const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  const int2* nu_t = P.near_u + (size_t)task * P.far_nclus;
+  const size_t ns = (size_t)P.far_nclus;
+  double f[Q], acc[Q];
+  int jj[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int g = q / R;
+    jj[q] = gv.start_idx + (b0 + g) * 64 * R + lane + 64 * (q % R);
+    const bool has = nr[g].y > nr[g].x || nr[g].w > nr[g].z;   // runs are planned for whole blocks only
+    f[q] = has ? gv.freq[jj[q]] : 0.0;
+    acc[q] = 0.0;
+  }
+  unsigned long long pairs = 0;
+  for (int Lb = Lmin; Lb < Lmax; Lb += kNearBatch) {
+    const int nb = min(kNearBatch, Lmax - Lb);
+    wave_lds_handover();                     // the previous batch's reads are done
+    {
+      const int e = lane & (kNearBatch - 1), half = lane / kNearBatch;
+      if (e < nb) {
+        const int L = Lb + e;
+#pragma unroll
+        for (int k = half; k < kFarP; k += 2) s_q[wave][k][e] = recs[(size_t)(kFarHead + k) * ns + L];
+        if (half == 0) {
+          s_c[wave][e] = P.far_geom[L].x;
+          s_r[wave][e] = recs[L];
+        } else {
+          s_u[wave][e] = nu_t[L];
+        }
+      }
+    }
+    wave_lds_handover();
+    for (int e = 0; e < nb; ++e) {
+      const int L = Lb + e;
+      const double c = s_c[wave][e], r = s_r[wave][e];
+      const int2 u = s_u[wave][e];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (!((L >= nr[g].x && L < nr[g].y) || (L >= nr[g].z && L < nr[g].w))) continue;
+#pragma unroll
+        for (int q = g * R; q < (g + 1) * R; ++q) {
+          if (jj[q] < u.x || jj[q] >= u.y) {
+            const double id = rcp_nr(f[q] - c);
+            const double t = r * id;
+            double v = s_q[wave][kFarP - 1][e];
+#pragma unroll
+            for (int m = kFarP - 2; m >= 0; --m) v = __builtin_fma(v, t, s_q[wave][m][e]);
+            acc[q] += v * t * id;
+            pairs += kFarLeaf;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int g = q / R;
+    if (nr[g].y > nr[g].x || nr[g].w > nr[g].z) tk.out[jj[q]] += acc[q];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+  if (lane == 0 && pairs) atomicAdd(P.far_evals, pairs);
+}
+
 // Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
 // kernel's: c y (u + 1) / (u^2 + 2 y^2), u = x^2 + y^2 - 1/2, scaled by amp = S g).
 __device__ __forceinline__ double region1_line(double nu, double g, double y, float amp, double f) {
@@ -693,7 +885,7 @@ __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedT
 // Region I of the lines [a, z) (every one contributing) at bin f: leaves of the range
 // from the largest aligned clusters that serve f, partial leaves line by line.
 __device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, bool cplx,
-                            int a, int z, double f) {
+                            int a, int z, double f, int task, int jb) {
   if (z <= a) return 0.0;
   const int A = (a + kFarLeaf - 1) / kFarLeaf, B = z / kFarLeaf;
   if (A >= B) return region1_lines(P, tk, a, z, f);
@@ -707,7 +899,7 @@ __device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask&
       const int k = P.far_off[lev] + (i >> lev);
       const double4 gm = P.far_geom[k];
       const FarRecC rc{recs + k, (size_t)P.far_nclus};
-      if (!serves(gm, rc, cplx, f, f)) continue;
+      if (!serves(gm, rc, cplx, f, f, near_u_of(P, task, k), jb, jb)) continue;
       s += cluster_sum(rc, gm.x, cplx, f);
       i += 1 << lev;
       took = true;
@@ -873,7 +1065,8 @@ __device__ __forceinline__ void block_scan(double (&v)[N], double* s_tot) {
 // bound at every bin of the block, else through the tree bin by bin.
 template <int NW, bool CPLX, bool SUFFIX>
 __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, int lo,
-                             int hi, double f, double f_lo, double f_hi, int* s_cut, double* s_tot, double* s_red) {
+                             int hi, double f, double f_lo, double f_hi, int* s_cut, double* s_tot, double* s_red,
+                             int task, int jb) {
   constexpr int NB = 64 * NW;
   const int tid = threadIdx.x;
   __syncthreads();                                             // s_cut of an earlier call read
@@ -906,7 +1099,7 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
   if (!(rE > 0.0)) rE = 1.0;
   const double dist = fmax(fmax(f_lo - cE, cE - f_hi), 0.0);
   if (!series_ok<kRangeP>(rE, gE, glo, dist, CPLX))
-    return range_sum(P, tk, recs, CPLX, lo * kFarLeaf, hi * kFarLeaf, f);
+    return range_sum(P, tk, recs, CPLX, lo * kFarLeaf, hi * kFarLeaf, f, task, jb);
   double m[RangeMoments<CPLX>::N];
 #pragma unroll
   for (int i = 0; i < RangeMoments<CPLX>::N; ++i) m[i] = 0.0;
@@ -951,14 +1144,14 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
     const int A = (s0 + kFarLeaf - 1) / kFarLeaf;
     pairs += (unsigned long long)(er.y - s0);
     acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
-    acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red);
+    acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   if (er.z < er.w) {                                           // er.z is a leaf boundary
     const int p1 = first_above<true>(P, er.z, er.w, j);        // ib <= j before here
     const int D = p1 / kFarLeaf;
     pairs += (unsigned long long)(p1 - er.z);
     acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
-    acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, f_lo, f_hi, s_cut, s_tot, s_red);
+    acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   tk.out[j] += acc;
   // pairs of this wave, counted once
@@ -984,6 +1177,12 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int R>
+dim3 near_grid(const MixedPlan& plan) {
+  const int waves = (plan.nblocks + NearShape<R>::G - 1) / NearShape<R>::G;
+  return dim3((waves + 3) / 4, plan.ntasks);
+}
+
 hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   if (!plan.far || plan.ntasks <= 0) return hipSuccess;
   const dim3 grid((plan.nblocks + 3) / 4, plan.ntasks);
@@ -991,21 +1190,25 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   switch (R) {
     case 1:
       k_far_eval<1><<<grid, 256, 0, s>>>(plan);
+      if (plan.near) k_near_eval<1><<<near_grid<1>(plan), 256, 0, s>>>(plan);
       k_far_edges<1, false><<<grid_blocks, 64, 0, s>>>(plan);
       k_far_edges<1, true><<<grid_blocks, 64, 0, s>>>(plan);
       break;
     case 2:
       k_far_eval<2><<<grid, 256, 0, s>>>(plan);
+      if (plan.near) k_near_eval<2><<<near_grid<2>(plan), 256, 0, s>>>(plan);
       k_far_edges<2, false><<<grid_blocks, 128, 0, s>>>(plan);
       k_far_edges<2, true><<<grid_blocks, 128, 0, s>>>(plan);
       break;
     case 4:
       k_far_eval<4><<<grid, 256, 0, s>>>(plan);
+      if (plan.near) k_near_eval<4><<<near_grid<4>(plan), 256, 0, s>>>(plan);
       k_far_edges<4, false><<<grid_blocks, 256, 0, s>>>(plan);
       k_far_edges<4, true><<<grid_blocks, 256, 0, s>>>(plan);
       break;
     default:
       k_far_eval<8><<<grid, 256, 0, s>>>(plan);
+      if (plan.near) k_near_eval<8><<<near_grid<8>(plan), 256, 0, s>>>(plan);
       k_far_edges<8, false><<<grid_blocks, 512, 0, s>>>(plan);
       k_far_edges<8, true><<<grid_blocks, 512, 0, s>>>(plan);
       break;

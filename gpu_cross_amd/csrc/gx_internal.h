@@ -53,6 +53,13 @@ struct GridView {
   double dfreq;
 };
 
+// Centre bin of a line on the grid (nearest node), clamped far inside int range.
+__device__ __forceinline__ int centre_bin(double nu_if, const GridView& gv) {
+#pragma clang fp contract(off)
+  const double c = rint((nu_if - gv.start_nu) / gv.dfreq);
+  return (int)fmin(fmax(c, -1073741824.0), 1073741824.0);
+}
+
 // Resident line list, sorted by nu (SoA, FP64 as the reference's BaseManager takes it).
 struct LinesView {
   const double* nu;
@@ -115,6 +122,13 @@ constexpr double kFarCplxGamma = 0.05;  // points whose sampled Lorentz widths a
                                         // take complex-centred moments
 constexpr int kFarList = 128;        // clusters per block at most
 constexpr int kFarMaxLevels = 32;
+// Near leaves (real-centred points): a leaf whose lines' W4 cores and series-convergence zone
+// fit in one bin range U (its near range) has its lines evaluated by the core pass at every bin
+// of U (region I outside their cores) and is summed from its moments at the other bins of the
+// blocks whose near lines it holds.  U must lie within kmax of each line's centre bin; the core
+// pass rows get kNearMargin bins of room for the part of U beyond the cores.
+constexpr int kNearMargin = 8;
+constexpr double kNearRho = 0.42;    // leaf series used where rho = r / |f - nu_c| <= this
 
 // A mixed-precision launch over ntasks points sharing one grid, one sorted
 // line list and one set of block ranges.  Work items are (task, block, segment).
@@ -164,6 +178,14 @@ struct MixedPlan {
   int* far_list;           // [ntasks][nblocks][kFarList]: clusters of each block, level << 24 | index
   int* far_cnt;            // [ntasks][nblocks]
   unsigned long long* far_evals;   // line-bin pairs taken from moments
+  // near leaves (see kNearMargin): near == 0 leaves every line's exclusion range at its core
+  int near;
+  int2* near_u;            // [ntasks][far_nclus]: near range U of a leaf (empty when the leaf is not
+                           // near-eligible), the hull of its leaves' ranges for a tree cluster
+  int4* near_rng;          // [ntasks][nblocks]: runs [x, y) and [z, w) of near leaves (leaf indices)
+                           // summed from moments per bin by k_near_eval
+  unsigned long long* core_evals;  // kEvalSlots counters of core-pass evaluations (may be NULL):
+                                   // the core pass adds the bins of U beyond each line's core
 };
 
 // One request of a batched FP64 launch (launch_f64_batch): its constants, grid, line slice

@@ -318,13 +318,6 @@ __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restric
   if (core_evals) block_count(w, core_evals);
 }
 
-// Centre bin of a line on the grid (nearest node), clamped far inside int range.
-__device__ __forceinline__ int centre_bin(double nu_if, const GridView& gv) {
-#pragma clang fp contract(off)
-  const double c = rint((nu_if - gv.start_nu) / gv.dfreq);
-  return (int)fmin(fmax(c, -1073741824.0), 1073741824.0);
-}
-
 // ---------------------------------------------------------------------------
 // Slice of the sorted list that may touch bin block b: nu within the block's
 // frequency span widened by the largest cutoff and two bins.  The exact
@@ -692,7 +685,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
         const double cj = rint((nu - gv.start_nu) * inv_dfreq);     // nearest bin (r below is consistent)
         const double fc = gv.start_nu + cj * gv.dfreq;
         const int ib = P.ib[i], ie = P.ie[i];
-        const int2 core = tk.core[i];
+        int2 core = tk.core[i];                 // bins left to the core pass: the W4 core, or the
+        if (P.near) {                           // near range of the line's leaf when it has one
+          const int2 u = P.near_u[(size_t)task * P.far_nclus + i / kFarLeaf];
+          if (u.y > u.x) core = u;
+        }
         const float amp = tk.amp[i];
         // Power-of-two rescaling x' = sigma x keeps x'^4 far from FP32 overflow for lines
         // whose window spans huge x (tiny nu => huge 1/alpha); region I is invariant under
@@ -1006,9 +1003,12 @@ constexpr int kCoreDenseLines = 2 * kCoreLaneLines;
 // The chunks [l0, l1) of a core workgroup with RL lines per lane (lane L takes lines
 // chunk + RL L + r).  The lane's base bin cb is its first line's centre bin; all its lines
 // are evaluated at bin cb + k (each with its own x, so a line whose centre bin differs from cb
-// only shifts its core range).  kFreqLds: grid values of the row staged in LDS, interleaved
-// with the sums (entry i at s_core[2i], its frequency at s_core[2i + 1]); else the sums alone
-// and the frequencies read from HBM.
+// only shifts its core range).  A line of a near leaf (one with a near range U, which holds
+// the W4 cores of all its lines) is evaluated at every bin of U: region I where |x| + y >= 15
+// (the FP64 decision, so exactly its bins outside its core), regions II-IV elsewhere; the far
+// field and the region-I kernel take its other bins.  kFreqLds: grid values of the row staged
+// in LDS, interleaved with the sums (entry i at s_core[2i], its frequency at s_core[2i + 1]);
+// else the sums alone and the frequencies read from HBM.
 template <bool kFreqLds, int RL>
 __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask& tk, double* s_core, int base,
                                             int l0, int l1) {
@@ -1022,6 +1022,7 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
     CoreLine L[RL];
     int cb = 0x7fffffff - 64 + lane;            // lanes without lines: singleton segments out of reach
     int lo = 0x3fffffff, hi = -0x3fffffff;
+    unsigned long long extra = 0;              // near-range bins beyond the lines' cores
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int l = first + r;
@@ -1038,7 +1039,14 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         L[r].g = tk.g[l];
         L[r].y = tk.y[l];
         L[r].amp = tk.amp[l];
-        const int2 core = tk.core[l];
+        int2 core = tk.core[l];
+        if (P.near) {                            // the near range of the line's leaf (it holds the
+          const int2 u = P.near_u[(size_t)blockIdx.y * P.far_nclus + l / kFarLeaf];   // line's core)
+          if (u.y > u.x) {
+            extra += (unsigned long long)(u.y - u.x) - (unsigned long long)max(core.y - core.x, 0);
+            core = u;
+          }
+        }
         if (core.y > core.x) {
           L[r].lo = max(core.x, base) - cb;
           L[r].hi = min(core.y, base + span) - cb;
@@ -1046,6 +1054,10 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         lo = min(lo, L[r].lo);
         hi = max(hi, L[r].hi);
       }
+    }
+    if (P.near && P.core_evals) {
+      extra = wave_sum_u64(extra);
+      if (lane == 0 && extra) atomicAdd(&P.core_evals[blockIdx.x & (kEvalSlots - 1)], extra);
     }
     if (!__ballot(hi > lo)) continue;
 #pragma unroll
@@ -1076,7 +1088,7 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         for (int r = 0; r < RL; ++r)
           if (k >= L[r].lo && k < L[r].hi) {
             const double x = fabs(L[r].nu - f) * L[r].g;
-            v = __builtin_fmaf(L[r].amp, w4_core_mixed(x, L[r].y, L[r].r2), v);
+            v = __builtin_fmaf(L[r].amp, w4_near_mixed(x, L[r].y, L[r].r2), v);
           }
       }
       v = seg_scan(v, m1, m2, m4, m8, m15, m31);

@@ -14,7 +14,7 @@
 // gx_table_run.  Points are admitted by free HBM: when the per-point line
 // data of all points does not fit, the table runs in point batches, each its
 // own persistent launch inside the same graph.  An autotuner times the
-// launch shape (bins per lane, work split) on the first points once.
+// launch shape (bins per block, work split) on the table itself once per shape.
 //
 // The FP64 precision of a table is the plain per-point path: one intensity,
 // ranges and accumulation launch per point, the denominator of the speedup
@@ -172,6 +172,7 @@ int prepare_launches(gx_table* t) {
     if (rc0) return rc0;
     l.run.R = t->R;
     l.run.items_per_slot = t->ips;
+    l.run.core_counter = t->core_evals;
     // the launch's points are contiguous in the grouped constants k_prep_mixed reads
     std::vector<int> trow(l.count);
     mixed_temperature_rows(t->der.h_pcs.data() + g.first_const + l.first, l.count, trow.data());
@@ -192,38 +193,41 @@ int prepare_launches(gx_table* t) {
   return GX_OK;
 }
 
-// Time each launch shape on the first (up to 4) points of the first group and keep
-// the fastest; the choice is cached per (bins, lines, points per launch).
+// Time each launch shape on the table itself (every launch, best of two runs: the far field,
+// region I and the core pass share the work differently at each block size, and a table's
+// points differ widely in how their pairs split between them) and keep the fastest; the choice
+// is cached per (bins, lines, points per launch).
 int autotune(gx_table* t) {
   const int forced_R = env_R();                 // GX_MIXED_R restricts the choice to that block size
   const auto key = std::make_tuple(t->M, t->L->ls.n, t->launches.empty() ? 0 : t->launches[0].count, forced_R);
   auto it = g_tuned.find(key);
   if (it == g_tuned.end() && !t->launches.empty()) {
-    std::vector<Launch> saved;
-    saved.swap(t->launches);
-    t->launches.emplace_back();
-    t->launches[0].group = saved[0].group;
-    t->launches[0].first = 0;
-    t->launches[0].count = std::min(4, saved[0].count);
     float best = 1e30f;
-    int bR = 4, bI = 4;
-    int last_nseg = -1, last_R = -1;
+    int bR = 2, bI = 4;
+    int last_R = -1;
+    long long last_shape = -1;
     int rc = GX_OK;
-    for (int R : {1, 2, 4, 8}) {
-      if (forced_R && R != forced_R) continue;
+    const std::vector<int> cands = forced_R ? std::vector<int>{forced_R} : std::vector<int>{1, 2, 4};
+    for (int R : cands) {
       for (int ips : {4, 8, 2}) {
         t->R = R;
         t->ips = ips;
         if ((rc = prepare_launches(t))) break;
-        const int nseg = t->launches[0].run.plan.nseg;
-        if (R == last_R && nseg == last_nseg) continue;   // same launch shape as a timed one
+        long long shape = 0;                            // the launch shape: segments of every launch
+        for (const Launch& l : t->launches) shape = shape * 17 + l.run.plan.nseg;
+        if (R == last_R && shape == last_shape) continue;   // same launch shape as a timed one
         last_R = R;
-        last_nseg = nseg;
-        if ((rc = enqueue(t, true))) break;
-        hipError_t e;
-        if ((e = hipStreamSynchronize(t->stream))) { rc = hip_fail(e, "autotune"); break; }
-        float ms = 0;
-        if ((e = hipEventElapsedTime(&ms, t->ev[1], t->ev[2]))) { rc = hip_fail(e, "autotune events"); break; }
+        last_shape = shape;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2 && !rc; ++rep) {
+          hipError_t e;
+          if ((rc = enqueue(t, true))) break;
+          if ((e = hipStreamSynchronize(t->stream))) { rc = hip_fail(e, "autotune"); break; }
+          float m = 0;
+          if ((e = hipEventElapsedTime(&m, t->ev[1], t->ev[2]))) { rc = hip_fail(e, "autotune events"); break; }
+          ms = std::min(ms, m);
+        }
+        if (rc) break;
         if (ms < best) {
           best = ms;
           bR = R;
@@ -232,8 +236,6 @@ int autotune(gx_table* t) {
       }
       if (rc) break;
     }
-    t->launches[0].run.release();
-    t->launches.swap(saved);
     if (rc) return rc;
     g_tuned[key] = {bR, bI};
     it = g_tuned.find(key);

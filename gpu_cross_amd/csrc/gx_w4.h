@@ -220,4 +220,17 @@ __device__ __forceinline__ float w4_core_mixed(double xd, double yd, const W4Reg
   return (float)((double)ex - tpq);
 }
 
+// W4 at a bin of a line's near range in the mixed core pass: region I (|x| + y >= 15, the FP64
+// decision) in its closed real form in FP32, c y (u + 1) / (u^2 + 2 y^2) with u = x^2 + y^2 - 1/2
+// (no cancellation there: u > 100; x stays below 1e6 in a near range, so u^2 stays finite), and
+// regions II-IV as w4_core_mixed.
+__device__ __forceinline__ float w4_near_mixed(double xd, double yd, const W4Region2& r2) {
+  if (w4_sum(xd, yd) >= 15.0) {
+    const float x = (float)xd, y = (float)yd;
+    const float u = __builtin_fmaf(x, x, __builtin_fmaf(y, y, -0.5f));
+    return (0.5641896f * y) * (u + 1.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, 2.0f * (y * y)));
+  }
+  return w4_core_mixed(xd, yd, r2);
+}
+
 }  // namespace gx
