@@ -261,6 +261,51 @@ def test_voigt_mixed_far_field(voigt_lines, monkeypatch, T, P):
     assert t_on.far_evals + t_on.core_evals < t_on.evals
 
 
+@pytest.fixture(scope="module")
+def dense_lines(voigt_lines):
+    """300k lines of the Voigt list with their centres mapped affinely into 6000-6100 cm^-1
+    (~3000 per cm^-1, 64 lines within ~4 bins of 0.005 cm^-1, as dense as the 1e8-line
+    workloads), so that leaves are narrow enough to have near ranges."""
+    ll = voigt_lines
+    L = ll.subset(np.nonzero(ll.nu > 1000.0)[0][:300_000])
+    lo, hi = L.nu.min(), L.nu.max()
+    L.nu = 6000.0 + (L.nu - lo) * (100.0 / (hi - lo))
+    return L
+
+
+@pytest.mark.parametrize("R,T,P", [("1", 300.0, 1e-4), ("1", 3000.0, 0.025), ("2", 1000.0, 0.4), ("1", 1500.0, 0.1),
+                                   ("4", 2000.0, 1e-3)])
+def test_voigt_mixed_near_leaves(voigt_lines, dense_lines, monkeypatch, R, T, P):
+    """Near leaves (the default; GX_NEAR=0 switches them off): leaves whose lines' cores reach
+    a block are summed from their moments at its other bins and their near-range bins go to the
+    core pass.  Both forms against the FP64 oracle; with near leaves more pairs come from
+    moments and more from the core pass, and the two forms agree far inside the tolerance."""
+    monkeypatch.setenv("GX_MIXED_R", R)
+    ll, L = voigt_lines, dense_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    freq, d = make_grid(6040.0, 6060.0, 4000)
+    gam = L.gamma_at(P)
+    cut = 3.0
+    print(f"voigt mixed near leaves R={R} T={T} P={P} lines={L.n}", flush=True)
+    on, t_on = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                           cutoff=cut)
+    monkeypatch.setenv("GX_NEAR", "0")
+    off, t_off = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                             cutoff=cut)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=cut)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    e_on, e_off = rel_err(on, ref), rel_err(off, ref)
+    direct_on = t_on.evals - t_on.far_evals - t_on.core_evals
+    direct_off = t_off.evals - t_off.far_evals - t_off.core_evals
+    print(f"  region-I pairs one by one: {direct_on:.3e} with near leaves, {direct_off:.3e} without; "
+          f"max rel err {e_on:.3e} / {e_off:.3e}", flush=True)
+    assert e_on <= MIXED_TOL and e_off <= MIXED_TOL
+    assert t_on.evals == t_off.evals
+    assert t_on.far_evals > t_off.far_evals and t_on.core_evals > t_off.core_evals
+    assert direct_on < direct_off
+    assert rel_err(on, off) <= 2e-6
+
+
 @pytest.mark.parametrize("R,T,P", [("1", 3000.0, 1e-3), ("1", 1000.0, 100.0), ("2", 300.0, 10.0)])
 def test_voigt_mixed_far_field_small_blocks(voigt_lines, monkeypatch, R, T, P):
     """The far field with blocks of 64 and 128 bins (real and complex centres) against the
