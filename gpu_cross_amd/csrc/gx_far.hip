@@ -52,6 +52,15 @@ constexpr double kKappa = 0.5 * 0.5641896;             // c / 2, c of humlic's r
 
 __device__ __forceinline__ double nu_if(double v) { return v == 0.0 ? 1e-6 : v; }
 
+// 1/d for d != 0: the hardware reciprocal refined by two Newton steps (the series are
+// approximations held to a bound far above this rounding).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+
+
 // One cluster's record inside a task's field-major record array: rc[f] is field f.
 template <class T>
 struct FarRec {
@@ -522,7 +531,7 @@ __device__ __forceinline__ int2 near_u_of(const MixedPlan& P, int task, int k) {
 // Sum of a cluster (record rc, centre nu_c) at bin f.
 __device__ __forceinline__ double cluster_sum(FarRecC rc, double nu_c, bool cplx, double f) {
   if (!cplx) {
-    const double id = 1.0 / (f - nu_c);
+    const double id = rcp_nr(f - nu_c);
     const double t = rc[0] * id;
     double q = rc[kFarHead + kFarP - 1];
 #pragma unroll
@@ -530,7 +539,7 @@ __device__ __forceinline__ double cluster_sum(FarRecC rc, double nu_c, bool cplx
     return q * t * id;
   }
   const double dr = f - nu_c, di = -rc[5];                    // D = f - c
-  const double inv = 1.0 / (dr * dr + di * di);
+  const double inv = rcp_nr(dr * dr + di * di);
   const double ir = dr * inv, ii = -di * inv;                 // 1/D
   const double tr = rc[0] * ir, ti = rc[0] * ii;              // r/D
   double sr = rc[kFarHead + 2 * (kFarP - 1)], si = rc[kFarHead + 2 * (kFarP - 1) + 1];
@@ -670,181 +679,228 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
   if (lane == 0) P.edge_rng[slot] = er;
 }
 
-// Far-field sums of one (task, block) per wave, added to the task's grid.  A cluster's record
-// is fetched by one load over the lanes (lane i takes field i; the fields lie in different
-// rows of the field-major array), staged in LDS and read back as broadcasts; the next
-// cluster's fetch is in flight while the current one is summed.
-template <int R>
-__global__ __launch_bounds__(256) void k_far_eval(MixedPlan P) {
-  __shared__ double s_rc[4][kFarRec];
+// Far-field sums of one (task, block) per wave, added to the task's grid, one instantiation
+// per moment kind.  NL clusters are summed side by side (BL lanes each, Q bins per lane: 8
+// for real centres, 4 for complex ones): their records are fetched together (each lane loads
+// a few fields of them; the fields lie in different rows of the field-major array), staged in
+// LDS and read back moment by moment for all the lane's bins, and each bin's NL partial sums
+// are added in a fixed order at the end.
+template <int R, bool CPLX>
+struct FarEvalShape {
+  static constexpr int Q = (CPLX ? 4 : 8) > R ? (CPLX ? 4 : 8) : R;   // bins per lane
+  static constexpr int BL = 64 * R / Q;                                // lanes per cluster
+  static constexpr int NL = 64 / BL;                                   // clusters at a time
+};
+
+template <int R, bool CPLX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_far_eval(MixedPlan P) {
+  using S = FarEvalShape<R, CPLX>;
+  constexpr int NL = S::NL, BL = S::BL, Q = S::Q;
+  constexpr int nf = CPLX ? kFarRec : kFarHead + kFarP;
+  __shared__ double s_rc[4][NL][nf];
+  __shared__ double s_c[4][NL];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.x * 4 + wave;
   if (b >= P.nblocks) return;
   const int task = blockIdx.y;
-  const size_t slot = (size_t)task * P.nblocks + b;
-  const int cnt = P.far_cnt[slot];
+  if ((P.far_kind[task] != 0) != CPLX) return;               // the other kind's launch
+  const size_t slot_b = (size_t)task * P.nblocks + b;
+  const int cnt = P.far_cnt[slot_b];
   if (cnt == 0) return;
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
   const int w0 = gv.start_idx + b * 64 * R;
-  const int* list = P.far_list + slot * kFarList;
+  const int* list = P.far_list + slot_b * kFarList;
   const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const size_t ns = (size_t)P.far_nclus;
-  double f[R], acc[R];
+  const int slot = lane / BL, bl = lane % BL;
+  double f[Q], acc[Q];
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    f[q] = gv.freq[w0 + lane + 64 * q];       // far clusters are planned for whole blocks only
+  for (int q = 0; q < Q; ++q) {
+    f[q] = gv.freq[w0 + bl + BL * q];       // far clusters are planned for whole blocks only
     acc[q] = 0.0;
   }
-  const bool cplx = P.far_kind[task] != 0;
-  const int nf = cplx ? kFarRec : kFarHead + kFarP;
-  auto cluster = [&](int e) {
-    const int ent = list[e];
-    return P.far_off[ent >> 24] + (ent & 0xffffff);
-  };
-  double* my = s_rc[wave];
-  int k = cluster(0);
-  double v = lane < nf ? recs[(size_t)lane * ns + k] : 0.0;
-  double c = P.far_geom[k].x;
-  for (int e = 0; e < cnt; ++e) {
-    wave_lds_handover();                     // the previous cluster's reads are done
-    if (lane < nf) my[lane] = v;
-    wave_lds_handover();
-    const double cc = c;
-    if (e + 1 < cnt) {                       // next record in flight
-      k = cluster(e + 1);
-      v = lane < nf ? recs[(size_t)lane * ns + k] : 0.0;
-      c = P.far_geom[k].x;
+  for (int e0 = 0; e0 < cnt; e0 += NL) {
+    const int ne = min(NL, cnt - e0);
+    wave_lds_handover();                     // the previous clusters' reads are done
+    for (int idx = lane; idx < ne * nf; idx += 64) {
+      const int sl = idx / nf, fld = idx - sl * nf;
+      const int ent = list[e0 + sl];
+      const int k = P.far_off[ent >> 24] + (ent & 0xffffff);
+      s_rc[wave][sl][fld] = recs[(size_t)fld * ns + k];
+      if (fld == 0) s_c[wave][sl] = P.far_geom[k].x;
     }
-    const FarRecC rc{my, 1};
-    if (cplx) {
+    wave_lds_handover();
+    if (slot < ne) {
+      const double* rc = &s_rc[wave][slot][0];
+      const double c = s_c[wave][slot];
+      if (CPLX) {
+        // cluster_sum's complex series, moment by moment over the lane's bins
+        double ir[Q], ii[Q], tr[Q], ti[Q], sr[Q], si[Q];
 #pragma unroll
-      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, cc, true, f[q]);
-    } else {
+        for (int q = 0; q < Q; ++q) {
+          const double dr = f[q] - c, di = -rc[5];
+          const double inv = rcp_nr(dr * dr + di * di);
+          ir[q] = dr * inv;
+          ii[q] = -di * inv;
+          tr[q] = rc[0] * ir[q];
+          ti[q] = rc[0] * ii[q];
+          sr[q] = rc[kFarHead + 2 * (kFarP - 1)];
+          si[q] = rc[kFarHead + 2 * (kFarP - 1) + 1];
+        }
 #pragma unroll
-      for (int q = 0; q < R; ++q) acc[q] += cluster_sum(rc, cc, false, f[q]);
+        for (int m = kFarP - 2; m >= 0; --m) {
+          const double mr = rc[kFarHead + 2 * m], mi = rc[kFarHead + 2 * m + 1];
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            const double nr = (sr[q] * tr[q] - si[q] * ti[q]) + mr;
+            const double ni = (sr[q] * ti[q] + si[q] * tr[q]) + mi;
+            sr[q] = nr;
+            si[q] = ni;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] += sr[q] * ii[q] + si[q] * ir[q];
+      } else {
+        // cluster_sum's real series, moment by moment over the lane's bins
+        double id[Q], t[Q], v[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          id[q] = rcp_nr(f[q] - c);
+          t[q] = rc[0] * id[q];
+          v[q] = rc[kFarHead + kFarP - 1];
+        }
+#pragma unroll
+        for (int m = kFarP - 2; m >= 0; --m) {
+          const double a = rc[kFarHead + m];
+#pragma unroll
+          for (int q = 0; q < Q; ++q) v[q] = v[q] * t[q] + a;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] += v[q] * t[q] * id[q];
+      }
     }
   }
 #pragma unroll
-  for (int q = 0; q < R; ++q) tk.out[w0 + lane + 64 * q] += acc[q];
-}
-
-// 1/d for d != 0: the hardware reciprocal refined by two Newton steps (the series are
-// approximations held to a bound far above this rounding).
-__device__ __forceinline__ double rcp_nr(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
-  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  for (int d = BL; d < 64; d <<= 1)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] += __shfl_xor(acc[q], d);
+  if (slot == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) tk.out[w0 + bl + BL * q] += acc[q];
+  }
 }
 
 // Leaves staged per pass of k_near_eval.
 constexpr int kNearBatch = 32;
 
-// Blocks of 64 R bins per wave of k_near_eval: neighbouring blocks share most of their near
-// leaves, so one staging of a leaf's moments in LDS serves four bins per lane.
+// Lane layout of k_near_eval for blocks of 64 R bins: 8 / R leaves side by side, 8 R lanes per
+// leaf, 8 bins per lane (bin lane b takes bins b + 8 R q, q < 8), so one LDS read of a moment
+// serves eight series steps of its lane.
 template <int R>
 struct NearShape {
-  static constexpr int G = R >= 4 ? 1 : 4 / R;
-  static constexpr int Q = R * G;   // bins per lane
+  static constexpr int NL = 8 / R;          // leaves at a time
+  static constexpr int BL = 64 / NL;        // lanes per leaf
+  static constexpr int Q = 64 * R / BL;     // bins per lane (8)
 };
 
-// Near leaves of G neighbouring (task, block) pairs per wave (the runs k_far_plan left between
-// the far clusters and each block's direct lines): bin j of block b adds a leaf of b's runs
-// wherever j is outside the leaf's near range (the core pass evaluates the leaf's lines at the
-// bins inside it).  The leaves of the blocks' runs are consecutive, so each field of
-// kNearBatch leaves is one coalesced load (field-major records), staged in LDS and read back
-// as broadcasts; a leaf outside a block's runs is skipped for that block (a scalar branch).
+// Near leaves of one (task, block) per wave (the runs k_far_plan left between the far clusters
+// and the block's direct lines): bin j adds a leaf's real series wherever j is outside the
+// leaf's near range (the core pass evaluates the leaf's lines at the bins inside it).  The
+// leaves of a run are consecutive, so each field of kNearBatch leaves is one coalesced load
+// (field-major records), staged in LDS; NL leaves are summed side by side and their lanes'
+// sums added in a fixed order at the end.
 template <int R>
-__global__ __launch_bounds__(256) void k_near_eval(MixedPlan P) {
-  constexpr int G = NearShape<R>::G, Q = NearShape<R>::Q;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_near_eval(MixedPlan P) {
+  using S = NearShape<R>;
+  constexpr int NL = S::NL, BL = S::BL, Q = S::Q;
   __shared__ double s_q[4][kFarP][kNearBatch];
   __shared__ double s_c[4][kNearBatch], s_r[4][kNearBatch];
   __shared__ int2 s_u[4][kNearBatch];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int b0 = (blockIdx.x * 4 + wave) * G;
-  if (b0 >= P.nblocks) return;
+  const int b = blockIdx.x * 4 + wave;
+  if (b >= P.nblocks) return;
   const int task = blockIdx.y;
-  int4 nr[G];
-  int Lmin = 0x7fffffff, Lmax = -1;
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    nr[g] = b0 + g < P.nblocks ? P.near_rng[(size_t)task * P.nblocks + b0 + g] : make_int4(0, 0, 0, 0);
-    nr[g].x = __builtin_amdgcn_readfirstlane(nr[g].x);
-    nr[g].y = __builtin_amdgcn_readfirstlane(nr[g].y);
-    nr[g].z = __builtin_amdgcn_readfirstlane(nr[g].z);
-    nr[g].w = __builtin_amdgcn_readfirstlane(nr[g].w);
-    if (nr[g].y > nr[g].x) {
-      Lmin = min(Lmin, nr[g].x);
-      Lmax = max(Lmax, nr[g].y);
-    }
-    if (nr[g].w > nr[g].z) {
-      Lmin = min(Lmin, nr[g].z);
-      Lmax = max(Lmax, nr[g].w);
-    }
-  }
-  if (Lmin >= Lmax) return;
+  int4 nr = P.near_rng[(size_t)task * P.nblocks + b];
+  nr.x = __builtin_amdgcn_readfirstlane(nr.x);
+  nr.y = __builtin_amdgcn_readfirstlane(nr.y);
+  nr.z = __builtin_amdgcn_readfirstlane(nr.z);
+  nr.w = __builtin_amdgcn_readfirstlane(nr.w);
+  if (nr.x >= nr.y && nr.z >= nr.w) return;
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
   const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const int2* nu_t = P.near_u + (size_t)task * P.far_nclus;
   const size_t ns = (size_t)P.far_nclus;
+  const int slot = lane / BL, bl = lane % BL;
+  const int w0 = gv.start_idx + b * 64 * R;
   double f[Q], acc[Q];
   int jj[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int g = q / R;
-    jj[q] = gv.start_idx + (b0 + g) * 64 * R + lane + 64 * (q % R);
-    const bool has = nr[g].y > nr[g].x || nr[g].w > nr[g].z;   // runs are planned for whole blocks only
-    f[q] = has ? gv.freq[jj[q]] : 0.0;
+    jj[q] = w0 + bl + BL * q;
+    f[q] = gv.freq[jj[q]];                   // near runs are planned for whole blocks only
     acc[q] = 0.0;
   }
   unsigned long long pairs = 0;
-  for (int Lb = Lmin; Lb < Lmax; Lb += kNearBatch) {
-    const int nb = min(kNearBatch, Lmax - Lb);
-    wave_lds_handover();                     // the previous batch's reads are done
-    {
-      const int e = lane & (kNearBatch - 1), half = lane / kNearBatch;
-      if (e < nb) {
-        const int L = Lb + e;
+  for (int run = 0; run < 2; ++run) {
+    const int L0 = run ? nr.z : nr.x, L1 = run ? nr.w : nr.y;
+    for (int Lb = L0; Lb < L1; Lb += kNearBatch) {
+      const int nb = min(kNearBatch, L1 - Lb);
+      wave_lds_handover();                   // the previous batch's reads are done
+      {
+        const int e = lane & (kNearBatch - 1), half = lane / kNearBatch;
+        if (e < nb) {
+          const int L = Lb + e;
 #pragma unroll
-        for (int k = half; k < kFarP; k += 2) s_q[wave][k][e] = recs[(size_t)(kFarHead + k) * ns + L];
-        if (half == 0) {
-          s_c[wave][e] = P.far_geom[L].x;
-          s_r[wave][e] = recs[L];
-        } else {
-          s_u[wave][e] = nu_t[L];
+          for (int k = half; k < kFarP; k += 2) s_q[wave][k][e] = recs[(size_t)(kFarHead + k) * ns + L];
+          if (half == 0) {
+            s_c[wave][e] = P.far_geom[L].x;
+            s_r[wave][e] = recs[L];
+          } else {
+            s_u[wave][e] = nu_t[L];
+          }
         }
       }
-    }
-    wave_lds_handover();
-    for (int e = 0; e < nb; ++e) {
-      const int L = Lb + e;
-      const double c = s_c[wave][e], r = s_r[wave][e];
-      const int2 u = s_u[wave][e];
+      wave_lds_handover();
+      for (int e0 = 0; e0 < nb; e0 += NL) {
+        const int e = e0 + slot;
+        if (e < nb) {
+          const double c = s_c[wave][e], r = s_r[wave][e];
+          const int2 u = s_u[wave][e];
+          // the leaf's series at the lane's bins outside its near range (zero weight inside),
+          // moment by moment over all of them
+          double id[Q], t[Q], v[Q];
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (!((L >= nr[g].x && L < nr[g].y) || (L >= nr[g].z && L < nr[g].w))) continue;
-#pragma unroll
-        for (int q = g * R; q < (g + 1) * R; ++q) {
-          if (jj[q] < u.x || jj[q] >= u.y) {
-            const double id = rcp_nr(f[q] - c);
-            const double t = r * id;
-            double v = s_q[wave][kFarP - 1][e];
-#pragma unroll
-            for (int m = kFarP - 2; m >= 0; --m) v = __builtin_fma(v, t, s_q[wave][m][e]);
-            acc[q] += v * t * id;
-            pairs += kFarLeaf;
+          for (int q = 0; q < Q; ++q) {
+            const bool out = jj[q] < u.x || jj[q] >= u.y;
+            pairs += out ? kFarLeaf : 0;
+            id[q] = out ? rcp_nr(f[q] - c) : 0.0;
+            t[q] = r * id[q];
+            v[q] = s_q[wave][kFarP - 1][e];
           }
+#pragma unroll
+          for (int m = kFarP - 2; m >= 0; --m) {
+            const double a = s_q[wave][m][e];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) v[q] = __builtin_fma(v[q], t[q], a);
+          }
+#pragma unroll
+          for (int q = 0; q < Q; ++q) acc[q] += v[q] * t[q] * id[q];
         }
       }
     }
   }
+  // the NL leaf slots of each bin, added in a fixed order
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int g = q / R;
-    if (nr[g].y > nr[g].x || nr[g].w > nr[g].z) tk.out[jj[q]] += acc[q];
+  for (int d = BL; d < 64; d <<= 1)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] += __shfl_xor(acc[q], d);
+  if (slot == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) tk.out[jj[q]] += acc[q];
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
@@ -1179,8 +1235,7 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
 
 template <int R>
 dim3 near_grid(const MixedPlan& plan) {
-  const int waves = (plan.nblocks + NearShape<R>::G - 1) / NearShape<R>::G;
-  return dim3((waves + 3) / 4, plan.ntasks);
+  return dim3((plan.nblocks + 3) / 4, plan.ntasks);
 }
 
 hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
@@ -1189,25 +1244,29 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
   const dim3 grid_blocks(((unsigned)plan.nblocks * (unsigned)plan.ntasks + 7u) / 8u * 8u);   // k_far_edges
   switch (R) {
     case 1:
-      k_far_eval<1><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<1, false><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<1, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<1><<<near_grid<1>(plan), 256, 0, s>>>(plan);
       k_far_edges<1, false><<<grid_blocks, 64, 0, s>>>(plan);
       k_far_edges<1, true><<<grid_blocks, 64, 0, s>>>(plan);
       break;
     case 2:
-      k_far_eval<2><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<2, false><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<2, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<2><<<near_grid<2>(plan), 256, 0, s>>>(plan);
       k_far_edges<2, false><<<grid_blocks, 128, 0, s>>>(plan);
       k_far_edges<2, true><<<grid_blocks, 128, 0, s>>>(plan);
       break;
     case 4:
-      k_far_eval<4><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<4, false><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<4, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<4><<<near_grid<4>(plan), 256, 0, s>>>(plan);
       k_far_edges<4, false><<<grid_blocks, 256, 0, s>>>(plan);
       k_far_edges<4, true><<<grid_blocks, 256, 0, s>>>(plan);
       break;
     default:
-      k_far_eval<8><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<8, false><<<grid, 256, 0, s>>>(plan);
+      k_far_eval<8, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<8><<<near_grid<8>(plan), 256, 0, s>>>(plan);
       k_far_edges<8, false><<<grid_blocks, 512, 0, s>>>(plan);
       k_far_edges<8, true><<<grid_blocks, 512, 0, s>>>(plan);

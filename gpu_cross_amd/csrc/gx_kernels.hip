@@ -1022,7 +1022,6 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
     CoreLine L[RL];
     int cb = 0x7fffffff - 64 + lane;            // lanes without lines: singleton segments out of reach
     int lo = 0x3fffffff, hi = -0x3fffffff;
-    unsigned long long extra = 0;              // near-range bins beyond the lines' cores
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int l = first + r;
@@ -1042,10 +1041,7 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         int2 core = tk.core[l];
         if (P.near) {                            // the near range of the line's leaf (it holds the
           const int2 u = P.near_u[(size_t)blockIdx.y * P.far_nclus + l / kFarLeaf];   // line's core)
-          if (u.y > u.x) {
-            extra += (unsigned long long)(u.y - u.x) - (unsigned long long)max(core.y - core.x, 0);
-            core = u;
-          }
+          if (u.y > u.x) core = u;
         }
         if (core.y > core.x) {
           L[r].lo = max(core.x, base) - cb;
@@ -1054,10 +1050,6 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         lo = min(lo, L[r].lo);
         hi = max(hi, L[r].hi);
       }
-    }
-    if (P.near && P.core_evals) {
-      extra = wave_sum_u64(extra);
-      if (lane == 0 && extra) atomicAdd(&P.core_evals[blockIdx.x & (kEvalSlots - 1)], extra);
     }
     if (!__ballot(hi > lo)) continue;
 #pragma unroll
@@ -1123,6 +1115,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __syncthreads();
   const int l0 = P.starts[w * P.cpw];
   const int l1 = P.starts[min((w + 1) * P.cpw, P.ncent)];
+  if (P.near && P.core_evals) {
+    // the bins of near ranges beyond the lines' cores, counted apart from the offset loops
+    // (the lines' data is then in cache for them)
+    const int2* const near_u = P.near_u + (size_t)blockIdx.y * P.far_nclus;
+    unsigned long long extra = 0;
+    for (int l = l0 + lane; l < l1; l += 64) {
+      const int2 u = near_u[l / kFarLeaf];
+      if (u.y > u.x) {
+        const int2 core = tk.core[l];
+        extra += (unsigned long long)(u.y - u.x) - (unsigned long long)max(core.y - core.x, 0);
+      }
+    }
+    extra = wave_sum_u64(extra);
+    if (lane == 0 && extra) atomicAdd(&P.core_evals[blockIdx.x & (kEvalSlots - 1)], extra);
+  }
   if (l1 - l0 >= kCoreDenseLines * P.cpw)
     core_chunks<kFreqLds, kCoreLaneLines>(P, tk, s_core, base, l0, l1);
   else
