@@ -33,8 +33,8 @@ FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compe
 # table's autotuner picks R): HBM bytes per launch of the region-I kernel (2 FETCH_SIZE +
 # WRITE_SIZE, copy-calibrated), and the step's VALU issue use, sum over every kernel of one
 # 48-point table of SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x clock x step time).
-PROFILED_TRAFFIC = {("table_48", 1): 3.750e11, ("table_48", 2): 2.903e11, "ingest_1e8": None}
-PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.887, "step_ms": 494.4, "valu_insts": 2.60e11, "clock_ghz": 2.317}}
+PROFILED_TRAFFIC = {("table_48", 1): 1.370e11, "ingest_1e8": None}
+PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.825, "step_ms": 421.9, "valu_insts": 2.068e11, "clock_ghz": 2.32}}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -209,9 +209,11 @@ def bench_table(args, wl):
                      "traffic_source": "rocprofv3 2*FETCH_SIZE+WRITE_SIZE per launch (copy-calibrated), profiles/",
                      "kernel": "k_accum_voigt_mixed", "kernel_ms": r1_ms, "evals": r1_evals,
                      "flop_per_eval": FLOP_PER_EVAL},
-        "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV)", "ms": core_ms, "evals": core_evals,
+        "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV; region I of near ranges)", "ms": core_ms, "evals": core_evals,
                       "evals_per_s": core_evals / (core_ms * 1e-3) if core_ms > 0 else None},
-        "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, k_far_edges (FP32 leaf sums, FP64 tree and series)",
+        "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, "
+                                 "k_near_eval, k_far_edges (FP32 leaf sums, FP64 tree and series; near leaves summed "
+                                 "per bin)",
                       "ms": far_ms, "evals": far_evals, "share": far_evals / max(evals, 1)},
         "intensity_kernels_ms": eager_int,
         "step_valu_issue": dict(PROFILED_STEP_VALU.get(("table_48", tab.shape()[0]), {}),
