@@ -241,7 +241,7 @@ int Derived::reserve(size_t n) {
 
 int Derived::reserve_mixed(size_t n, int npoints, int ntemps) {
   if (n <= mcap && npoints <= mpoints && ntemps <= mtemps) return GX_OK;
-  hipFree(m.amp); hipFree(m.g); hipFree(m.y); hipFree(m.core);
+  hipFree(m.amp); hipFree(m.g); hipFree(m.ts); hipFree(m.core);
   m = MixedLineData{};
   const size_t c = std::max<size_t>({n, mcap, 1});
   const int np = std::max({npoints, mpoints, 1}), nt = std::max({ntemps, mtemps, 1});
@@ -250,7 +250,7 @@ int Derived::reserve_mixed(size_t n, int npoints, int ntemps) {
   mtemps = 0;
   const size_t rows_t = c * (size_t)nt, rows_p = c * (size_t)np;
   hipError_t e;
-  if ((e = hipMalloc(&m.amp, rows_t * 4)) || (e = hipMalloc(&m.g, rows_t * 8)) || (e = hipMalloc(&m.y, rows_p * 8)) ||
+  if ((e = hipMalloc(&m.amp, rows_t * 4)) || (e = hipMalloc(&m.g, rows_t * 8)) || (e = hipMalloc(&m.ts, rows_t * 8)) ||
       (e = hipMalloc(&m.core, rows_p * sizeof(int2))))
     return hip_fail(e, "hipMalloc(mixed line data)");
   m.stride = c;
@@ -276,9 +276,10 @@ int mixed_temperature_rows(const PointConsts* pcs, int n, int* row) {
   return n > 0 ? r + 1 : 0;
 }
 
-MixedTask mixed_task(const MixedLineData& m, int point, int trow, const PointConsts& pc, double* out) {
+MixedTask mixed_task(const MixedLineData& m, int point, int trow, const PointConsts& pc, const double* gamma,
+                     double* out) {
   const size_t ot = (size_t)trow * m.stride, op = (size_t)point * m.stride;
-  return MixedTask{m.amp + ot, m.g + ot, m.y + op, m.core + op, pc.dpwcoef, out};
+  return MixedTask{m.amp + ot, m.g + ot, m.ts + ot, gamma, m.core + op, pc.pressure_scale, pc.dpwcoef, out};
 }
 
 int Derived::set_points(const PointConsts* pcs, int npts, hipStream_t s) {
@@ -302,7 +303,7 @@ void Derived::release() {
   hipFree(d.S); hipFree(d.g); hipFree(d.y); hipFree(d.ib); hipFree(d.ie);
   d = DerivedF64{};
   cap = 0;
-  hipFree(m.amp); hipFree(m.g); hipFree(m.y); hipFree(m.core);
+  hipFree(m.amp); hipFree(m.g); hipFree(m.ts); hipFree(m.core);
   m = MixedLineData{};
   mcap = 0;
   mpoints = 0;
@@ -691,7 +692,7 @@ int run_point(const PointConsts& pc, const GridView& gv, const LineStore& ls, in
   if (mixed) {
     if ((rc = der.reserve_mixed((size_t)std::max(nlines, 1), 1, 1))) return rc;
     if ((rc = der.set_points(&pc, 1, s))) return rc;
-    MixedTask task = mixed_task(der.m, 0, 0, pc, intens);
+    MixedTask task = mixed_task(der.m, 0, 0, pc, lv.gamma, intens);
     mix.core_counter = d_core_evals;
     if ((rc = mix.prepare(&task, 1, nblocks, gv, nlines, s))) return rc;
   }

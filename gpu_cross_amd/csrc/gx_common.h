@@ -159,7 +159,8 @@ struct MixedRunner {
 // row[p] (may be NULL) gets point p's row.  Returns the number of rows.
 int mixed_temperature_rows(const PointConsts* pcs, int n, int* row);
 // Task of point `point` (temperature row trow) over the line data m, summing into out.
-MixedTask mixed_task(const MixedLineData& m, int point, int trow, const PointConsts& pc, double* out);
+MixedTask mixed_task(const MixedLineData& m, int point, int trow, const PointConsts& pc, const double* gamma,
+                     double* out);
 
 // HBM of the far-field buffers of a mixed launch (ntasks points, nlines lines, nblocks blocks).
 size_t far_field_bytes(int ntasks, long long nlines, int nblocks);

@@ -91,7 +91,8 @@ __global__ __launch_bounds__(256) void k_far_kind(MixedPlan P) {
   double gmin = 1e300;
   for (int k = threadIdx.x; k < 4096; k += 256) {
     const long long l = (long long)k * P.nlines / 4096;
-    gmin = fmin(gmin, tk.y[l] * (1.0 / tk.g[l]));
+    const double g = tk.g[l];
+    gmin = fmin(gmin, mixed_y(tk, (int)l, g) * (1.0 / g));
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) gmin = fmin(gmin, __shfl_xor(gmin, o));
@@ -175,7 +176,7 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
     double y = 0.0;
     float amp = 0.0f;
     if (act) {
-      y = tk.y[l];
+      y = mixed_y(tk, l, tk.g[l]);
       amp = tk.amp[l];
     }
     const double al = act ? nu * tk.dpwcoef : 0.0;               // alpha: g = 1 / (nu dpwcoef)
@@ -928,13 +929,17 @@ __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedT
     for (int i = 0; i < 8; ++i) {
       nu[i] = P.nu[l + i];
       g[i] = tk.g[l + i];
-      y[i] = tk.y[l + i];
+      y[i] = tk.gamma[l + i] * tk.pscale;
+      y[i] *= tk.ts[l + i];
       amp[i] = tk.amp[l + i];
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) s += region1_line(nu[i], g[i], y[i], amp[i], f);
+    for (int i = 0; i < 8; ++i) s += region1_line(nu[i], g[i], y[i] * g[i], amp[i], f);
   }
-  for (; l < z; ++l) s += region1_line(P.nu[l], tk.g[l], tk.y[l], tk.amp[l], f);
+  for (; l < z; ++l) {
+    const double g = tk.g[l];
+    s += region1_line(P.nu[l], g, mixed_y(tk, l, g), tk.amp[l], f);
+  }
   return s;
 }
 

@@ -81,15 +81,20 @@ struct DerivedF64 {
 };
 
 // Line data of the mixed-precision path.  What depends on the temperature only is kept
-// once per run of consecutive points at one temperature, [ntemps][stride]: S/alpha and
-// 1/alpha; per point, [npoints][stride]: y and the W4 core range, an integer bin range found
-// with the FP64 path's own test |nu - f_j|/alpha + y < 15.  1/alpha and y are the FP64 path's
-// own bits, so every W4 region decision is that path's.  16 B per line and point plus 12 B
-// per line and temperature.
+// once per run of consecutive points at one temperature, [ntemps][stride]: S/alpha, 1/alpha
+// and the Lorentz temperature factor (T_ref/T)^n; per point, [npoints][stride], only the W4
+// core range, an integer bin range found with the FP64 path's own test
+// |nu - f_j|/alpha + y < 15.  y = ((gamma P) (T_ref/T)^n) / alpha is formed where it is used
+// (mixed_y), from the resident gamma and these rows, in the FP64 path's order of
+// operations, so y and 1/alpha are that path's bits and every W4 region decision is that
+// path's.  8 B per line and point plus 20 B per line and temperature (11.3 B per line and
+// point over 8 temperatures and 48 points).
+constexpr size_t kMixedPointBytes = 8;    // per line and point: the core range
+constexpr size_t kMixedTempBytes = 20;    // per line and temperature: S/alpha, 1/alpha, (T_ref/T)^n
 struct MixedLineData {
   float* amp;      // [ntemps][stride] S / alpha (FP32)
   double* g;       // [ntemps][stride] 1 / alpha = 1 / (nu * dpwcoef)
-  double* y;       // [npoints][stride] gamma_T / alpha
+  double* ts;      // [ntemps][stride] (T_ref / T)^n
   int2* core;      // [npoints][stride] [k0, k1): bins of the window where |x| + y < 15 (regions II-IV)
   size_t stride;   // lines per row
 };
@@ -105,11 +110,20 @@ struct WindowArgs {
 struct MixedTask {
   const float* amp;      // its temperature's rows of MixedLineData
   const double* g;
-  const double* y;       // its own rows
-  const int2* core;
+  const double* ts;
+  const double* gamma;   // the resident line list's gamma (indexed as nu)
+  const int2* core;      // its own row
+  double pscale;         // the point's pressure_scale
   double dpwcoef;        // Doppler width per unit nu of this point
   double* out;
 };
+
+// y = gamma_T / alpha of line l of a mixed task: ((gamma pressure_scale) (T_ref/T)^n) g, each
+// product rounded as the FP64 path rounds it (k_line_prep), so the bits are that path's.
+__device__ __forceinline__ double mixed_y(const MixedTask& tk, int l, double g) {
+#pragma clang fp contract(off)
+  return ((tk.gamma[l] * tk.pscale) * tk.ts[l]) * g;
+}
 
 // Far field of region I: moments per cluster, leaf size, list length, levels.
 constexpr int kFarP = 24;            // moments per cluster

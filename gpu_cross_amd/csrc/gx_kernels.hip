@@ -252,7 +252,7 @@ __device__ __forceinline__ bool same_t(const PointConsts& a, const PointConsts& 
          a.ref_temp == b.ref_temp && a.temperature == b.temperature && a.convention == b.convention;
 }
 
-// The pressure-dependent part and the point's outputs: y and the core range.
+// The pressure-dependent part and the point's output: the core range (y as mixed_y forms it).
 __device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts& pc, const PrepT& t, double nu_if,
                                                                double gamma, int ib, int ie, const GridView& gv,
                                                                double inv_dfreq, const MixedLineData& out, int i,
@@ -275,9 +275,7 @@ __device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts
       k1 = z;
     }
   }
-  const size_t o = (size_t)p * out.stride + i;
-  out.y[o] = y;
-  out.core[o] = make_int2(k0, k1);
+  out.core[(size_t)p * out.stride + i] = make_int2(k0, k1);
   return (unsigned long long)(k1 - k0);
 }
 
@@ -286,6 +284,7 @@ __device__ __forceinline__ void prep_mixed_store_t(const PrepT& t, const MixedLi
   const size_t o = (size_t)tq * out.stride + i;
   out.amp[o] = (float)(t.S * t.g);
   out.g[o] = t.g;
+  out.ts[o] = t.tscale;
 }
 
 // One thread per line over all npts points: the line's record is read once, and the
@@ -681,7 +680,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
         const double nu = v0 == 0.0 ? 1e-6 : v0;
-        const double g = tk.g[i], y = tk.y[i];                      // k_prep_mixed: FP64 path's bits
+        const double g = tk.g[i], y = mixed_y(tk, i, g);            // the FP64 path's bits
         const double cj = rint((nu - gv.start_nu) * inv_dfreq);     // nearest bin (r below is consistent)
         const double fc = gv.start_nu + cj * gv.dfreq;
         const int ib = P.ib[i], ie = P.ie[i];
@@ -1036,7 +1035,7 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         L[r].nu = v0 == 0.0 ? 1e-6 : v0;
         if (r == 0) cb = centre_bin(L[r].nu, gv);
         L[r].g = tk.g[l];
-        L[r].y = tk.y[l];
+        L[r].y = mixed_y(tk, l, L[r].g);
         L[r].amp = tk.amp[l];
         int2 core = tk.core[l];
         if (P.near) {                            // the near range of the line's leaf (it holds the

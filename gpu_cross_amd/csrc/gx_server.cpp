@@ -92,7 +92,7 @@ struct Slot {
 
   size_t footprint() const {
     return freq_cap * 8 + out_cap * 8 + win_cap * 8 + der.cap * 32 +
-           der.mcap * ((size_t)der.mpoints * 16 + (size_t)der.mtemps * 12) +
+           der.mcap * ((size_t)der.mpoints * kMixedPointBytes + (size_t)der.mtemps * kMixedTempBytes) +
            (size_t)rng.cap * sizeof(int2) + run.bytes() + evals_cap * 8 + t64_cap * sizeof(F64Task) +
            items_cap * sizeof(F64Item) + bitems_cap * sizeof(int2) + partial_cap * 8;
   }
@@ -173,7 +173,8 @@ size_t batch_need(const gx_server* S, const Slot& sl, const std::vector<Req*>& b
   const size_t M = (size_t)r.npoints, B = batch.size();
   size_t need = std::max(sl.freq_cap, M) * 8 + std::max(sl.out_cap, M * B) * 8 + std::max(sl.win_cap, nlines) * 8;
   if (mixed_req(r)) {
-    need += std::max(sl.der.mcap * ((size_t)sl.der.mpoints * 16 + (size_t)sl.der.mtemps * 12), nlines * B * 28);
+    need += std::max(sl.der.mcap * ((size_t)sl.der.mpoints * kMixedPointBytes + (size_t)sl.der.mtemps * kMixedTempBytes),
+                     nlines * B * (kMixedPointBytes + kMixedTempBytes));
     const size_t nb = (M + 1023) / 1024 + 1;
     need += nb * B * 16 * 1024 * sizeof(float2) / 4;   // partials, upper estimate
     need += (M + 4096) * B * 4 * sizeof(double);       // core-pass rows, upper estimate
@@ -412,7 +413,8 @@ int launch_batch(gx_server* S, Slot& sl) {
     std::vector<int> trow(B);
     mixed_temperature_rows(sl.h_pcs.data(), B, trow.data());
     std::vector<MixedTask> tasks(B);
-    for (int k = 0; k < B; ++k) tasks[k] = mixed_task(sl.der.m, k, trow[k], sl.h_pcs[k], sl.out + (size_t)k * M);
+    for (int k = 0; k < B; ++k)
+      tasks[k] = mixed_task(sl.der.m, k, trow[k], sl.h_pcs[k], lv.gamma, sl.out + (size_t)k * M);
     if ((rc = sl.run.prepare(tasks.data(), B, nblocks, gv, n, s))) return rc;
   }
   if ((e = hipEventRecord(sl.ev_start, s))) return hip_fail(e, "server: event");

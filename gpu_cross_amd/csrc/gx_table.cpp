@@ -179,7 +179,7 @@ int prepare_launches(gx_table* t) {
     std::vector<MixedTask> tasks(l.count);
     for (int k = 0; k < l.count; ++k) {
       const int p = g.points[l.first + k];
-      tasks[k] = mixed_task(t->der.m, k, trow[k], t->pcs[p], t->d_out + (size_t)p * t->M);
+      tasks[k] = mixed_task(t->der.m, k, trow[k], t->pcs[p], lines_view(t).gamma, t->d_out + (size_t)p * t->M);
     }
     const int nb = (t->M + mixed_bins_per_block(t->R) - 1) / mixed_bins_per_block(t->R);
     int rc = l.run.prepare(tasks.data(), l.count, nb, grid_view(t), (int)t->L->ls.n, t->stream);
@@ -439,7 +439,8 @@ int gx_table_create(gx_lines* L, const gx_grid* grid, const gx_point* pts, int n
     for (int p = 0; p < npts; ++p) rows = std::max(rows, core_rows_bytes(grid_view(t), t->pcs[p].dpwcoef));
     const int temps_all = mixed_temperature_rows(grouped.data(), npts, nullptr);
     const size_t nl = std::max<size_t>(n, 1);
-    const size_t per_point = 16 * nl + (12 * nl * (size_t)temps_all + npts - 1) / npts + rows + (size_t(64) << 10) +
+    const size_t per_point = kMixedPointBytes * nl + (kMixedTempBytes * nl * (size_t)temps_all + npts - 1) / npts + rows +
+                             (size_t(64) << 10) +
                              far_field_bytes(1, (long long)nl, t->M / 64 + 1);
     const size_t avail = free_b > headroom ? free_b - headroom : 0;
     t->batch = (int)std::min<size_t>((size_t)npts, avail / per_point);
