@@ -182,7 +182,10 @@ def bench_table(args, wl):
     evals = t.evals
     core_evals = t.core_evals
     far_evals, far_ms = t.far_evals, eager_far_ms
-    r1_evals = evals - core_evals - far_evals     # region-I pairs the direct kernel evaluated
+    r1_pairs = evals - core_evals - far_evals     # region-I pairs the region-I kernel summed
+    interp = t.interp_pairs                       # of them: interpolated from 16 nodes per 64 bins
+    r1_evals = r1_pairs - interp + interp // 4    # evaluations it made
+    log(f"region-I kernel: {r1_pairs:.4e} pairs, {interp:.4e} of them interpolated, {r1_evals:.4e} evaluations")
     wall = float(np.sum(walls))
     value = evals * args.steps / wall
     ms_step = wall / args.steps * 1e3
@@ -191,8 +194,9 @@ def bench_table(args, wl):
         "value": value, "unit": "line-bin pairs/s",
         "value_definition": "line-bin pairs (line i, bin j) with ib_i <= j < ie_i from the FP64 windows, "
                             "summed over the 48 grids, per second of table wall time; far_field.share of them "
-                            "are summed from multipole moments, the rest evaluated one by one "
-                            "(direct_evals_per_s)",
+                            "are summed from multipole moments, roofline.interpolated_pairs interpolated from "
+                            "16 node evaluations per line and 64-bin block, the rest evaluated one by one "
+                            "(direct_evals_per_s: evaluations made, node evaluations included)",
         "direct_evals_per_s": (r1_evals + core_evals) * args.steps / wall,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -208,7 +212,7 @@ def bench_table(args, wl):
                      "traffic": PROFILED_TRAFFIC.get(("table_48", tab.shape()[0])),
                      "traffic_source": "rocprofv3 2*FETCH_SIZE+WRITE_SIZE per launch (copy-calibrated), profiles/",
                      "kernel": "k_accum_voigt_mixed", "kernel_ms": r1_ms, "evals": r1_evals,
-                     "flop_per_eval": FLOP_PER_EVAL},
+                     "pairs": r1_pairs, "interpolated_pairs": interp, "flop_per_eval": FLOP_PER_EVAL},
         "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV; region I of near ranges)", "ms": core_ms, "evals": core_evals,
                       "evals_per_s": core_evals / (core_ms * 1e-3) if core_ms > 0 else None},
         "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, "
@@ -643,11 +647,12 @@ def main():
         m.close()
         return dict(wall=wall, acc_ms=float(np.mean(acc_ms)), int_ms=float(np.mean(int_ms)), evals=evals,
                     r1_ms=float(np.mean(r1_ms)), core_ms=float(np.mean(core_ms)), core_evals=t.core_evals,
-                    far_ms=float(np.mean(far_ms)), far_evals=t.far_evals, out=out / steps)
+                    far_ms=float(np.mean(far_ms)), far_evals=t.far_evals, interp=t.interp_pairs, out=out / steps)
 
     mixed = run(capi.PREC_MIXED, args.steps, args.warmup)
     evals = mixed["evals"]
-    r1_evals = evals - mixed["core_evals"] - mixed["far_evals"]   # region-I pairs the direct kernel evaluated
+    r1_pairs = evals - mixed["core_evals"] - mixed["far_evals"]   # region-I pairs of the region-I kernel
+    r1_evals = r1_pairs - mixed["interp"] + mixed["interp"] // 4  # evaluations it made (16 nodes per 64 bins)
     ms_step = mixed["wall"] / args.steps * 1e3
     value = evals * args.steps / mixed["wall"]
     res = {
@@ -664,7 +669,8 @@ def main():
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": FLOP_PER_EVAL * r1_evals / (mixed["r1_ms"] * 1e-3) / PEAK_FP32,
                      "traffic": None, "kernel": "k_accum_voigt_mixed", "kernel_ms": mixed["r1_ms"],
-                     "evals": r1_evals, "flop_per_eval": FLOP_PER_EVAL},
+                     "evals": r1_evals, "pairs": r1_pairs, "interpolated_pairs": mixed["interp"],
+                     "flop_per_eval": FLOP_PER_EVAL},
         "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV)", "ms": mixed["core_ms"],
                       "evals": mixed["core_evals"]},
         "far_field": {"ms": mixed["far_ms"], "evals": mixed["far_evals"], "share": mixed["far_evals"] / max(evals, 1)},

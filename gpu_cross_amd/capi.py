@@ -48,7 +48,7 @@ class Timing(ctypes.Structure):
     _fields_ = [("intensity_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
                 ("evals", ctypes.c_longlong), ("lines", ctypes.c_longlong), ("region1_ms", ctypes.c_float),
                 ("core_ms", ctypes.c_float), ("core_evals", ctypes.c_longlong), ("far_ms", ctypes.c_float),
-                ("far_evals", ctypes.c_longlong)]
+                ("far_evals", ctypes.c_longlong), ("interp_pairs", ctypes.c_longlong)]
 
 
 class HitranOpts(ctypes.Structure):

@@ -340,11 +340,21 @@ int MixedRunner::init() {
   }
   if (const char* env = std::getenv("GX_FAR")) far_on = std::atoi(env) != 0;
   if (const char* env = std::getenv("GX_NEAR")) near_on = std::atoi(env) != 0;
+  if (const char* env = std::getenv("GX_INTERP")) interp_on = std::atoi(env) != 0;
   hipError_t e = hipMalloc(&counter, sizeof(unsigned));
   if (e) return hip_fail(e, "hipMalloc(counter)");
   if ((e = hipMalloc(&far_evals, sizeof(unsigned long long))) || (e = hipEventCreate(&far_ev[0])) ||
       (e = hipEventCreate(&far_ev[1])))
     return hip_fail(e, "far-field counters");
+  {
+    std::vector<float> tab(kInterpTabNodes + kInterpNodes);
+    interp_table(tab.data());
+    if ((e = hipMalloc(&interp_tab, sizeof(float) * tab.size())) ||
+        (e = hipMalloc(&interp_cnt, sizeof(unsigned long long) * kEvalSlots)) ||
+        (e = hipMemcpy(interp_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice)) ||
+        (e = hipMemset(interp_cnt, 0, sizeof(unsigned long long) * kEvalSlots)))
+      return hip_fail(e, "interpolation table");
+  }
   int dev = 0;
   if ((e = hipGetDevice(&dev)) || (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)))
     return hip_fail(e, "device attributes");
@@ -363,6 +373,8 @@ void MixedRunner::release() {
   far_first = nullptr; far_first_cap = 0;
   hipFree(near_u); hipFree(near_rng);
   near_u = nullptr; near_rng = nullptr; near_u_cap = near_rng_cap = 0;
+  hipFree(interp_tab); hipFree(interp_cnt);
+  interp_tab = nullptr; interp_cnt = nullptr;
   for (auto& ev : far_ev) {
     if (ev) hipEventDestroy(ev);
     ev = nullptr;
@@ -387,6 +399,42 @@ int MixedRunner::far_time(float* ms, unsigned long long* evals) {
       (e = hipMemcpy(evals, far_evals, sizeof(unsigned long long), hipMemcpyDeviceToHost)))
     return hip_fail(e, "far-field timing");
   return GX_OK;
+}
+
+int MixedRunner::interp_pairs(long long* pairs) {
+  *pairs = 0;
+  if (!interp_cnt || plan.nitems <= 0) return GX_OK;
+  unsigned long long slots[kEvalSlots], n = 0;
+  hipError_t e;
+  if ((e = hipMemcpy(slots, interp_cnt, sizeof(slots), hipMemcpyDeviceToHost))) return hip_fail(e, "interpolation counters");
+  for (auto v : slots) n += v;
+  *pairs = (long long)n * 64;
+  return GX_OK;
+}
+
+// Barycentric interpolation from the Chebyshev-Lobatto nodes s_k = cos(pi k / (n - 1)) (weights
+// (-1)^k, halved at the ends) to the bin positions s_j = (j - 31.5) / 31.5 of a 64-bin block;
+// bins 0 and 63 are nodes n - 1 and 0.  FP64, rounded once.
+void interp_table(float* tab) {
+  constexpr int n = kInterpNodes;
+  double sk[n], wk[n];
+  for (int k = 0; k < n; ++k) {
+    sk[k] = k == 0 ? 1.0 : k == n - 1 ? -1.0 : std::cos(std::acos(-1.0) * k / (n - 1));
+    wk[k] = ((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == n - 1) ? 0.5 : 1.0);
+  }
+  for (int j = 0; j < 64; ++j) {
+    const double s = (j - 31.5) / 31.5;
+    int hit = -1;
+    for (int k = 0; k < n; ++k)
+      if (s == sk[k]) hit = k;
+    double d = 0.0;
+    if (hit < 0)
+      for (int k = 0; k < n; ++k) d += wk[k] / (s - sk[k]);
+    for (int k = 0; k < n; ++k)
+      tab[j * n + k] = hit >= 0 ? (k == hit ? 1.0f : 0.0f) : (float)((wk[k] / (s - sk[k])) / d);
+  }
+  // the anchored form's offset of each node: x = x(bin 32) + (32 - p_k) e, p_k = 31.5 (1 + s_k)
+  for (int k = 0; k < n; ++k) tab[kInterpTabNodes + k] = (float)(0.5 - 31.5 * sk[k]);
 }
 
 double chunk_max_abs_nu(const GridView& gv) {
@@ -537,6 +585,8 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
   plan.far = 0;
   plan.near = 0;
   plan.core_evals = core_counter;
+  plan.interp_tab = interp_on ? interp_tab : nullptr;
+  plan.interp_cnt = interp_cnt;
   if (far_plan) {
     int n = (plan.nlines + kFarLeaf - 1) / kFarLeaf, lev = 0, tot = 0;
     while (true) {
@@ -667,6 +717,8 @@ int MixedRunner::launch(const GridView& gv, const double* nu, const int2* ranges
     if (ev && (e = hipEventRecord(far_ev[1], s))) return hip_fail(e, "event");
     far_timed = ev != nullptr;
   }
+  if ((e = launch_fill(reinterpret_cast<double*>(interp_cnt), kEvalSlots, 0.0, s)))
+    return hip_fail(e, "interpolation counters");
   if (ev && (e = hipEventRecord(ev[0], s))) return hip_fail(e, "event");
   if ((e = launch_accum_mixed(plan, R, s))) return hip_fail(e, "mixed accumulate launch");
   if (ev && (e = hipEventRecord(ev[1], s))) return hip_fail(e, "event");
@@ -1004,6 +1056,7 @@ int gx_last_timing(gx_ctx* c, gx_timing* out) {
   out->core_evals = 0;
   out->far_ms = 0.0f;
   out->far_evals = 0;
+  out->interp_pairs = 0;
   if (c->last_mixed) {
     float r1 = 0, co = 0;
     if ((e = hipEventElapsedTime(&r1, c->ev[4], c->ev[5])) || (e = hipEventElapsedTime(&co, c->ev[5], c->ev[6])))
@@ -1018,6 +1071,8 @@ int gx_last_timing(gx_ctx* c, gx_timing* out) {
     const int frc = c->mix.far_time(&out->far_ms, &fe);
     if (frc) return frc;
     out->far_evals = (long long)fe;
+    const int irc = c->mix.interp_pairs(&out->interp_pairs);
+    if (irc) return irc;
   }
   return GX_OK;
 }

@@ -130,6 +130,10 @@ struct MixedRunner {
   int4* near_rng = nullptr;
   size_t near_rng_cap = 0;     // (task, block) slots
   unsigned long long* core_counter = nullptr;   // optional kEvalSlots counters of core-pass evaluations
+  // interpolated lines of the region-I kernel (GX_INTERP=0 switches them off)
+  bool interp_on = true;
+  float* interp_tab = nullptr;                  // interp_table(), uploaded by init
+  unsigned long long* interp_cnt = nullptr;     // kEvalSlots counters, cleared by each launch
   hipEvent_t far_ev[2] = {nullptr, nullptr};
   bool far_timed = false;      // the last launch recorded far_ev
   std::vector<MixedTask> h_tasks;
@@ -138,6 +142,9 @@ struct MixedRunner {
   // Device time of the last timed launch's far-field kernels and the line-bin pairs they
   // took from moments (call after the stream is synchronised).
   int far_time(float* ms, unsigned long long* evals);
+  // Line-bin pairs of the last launch that the region-I kernel formed by interpolation
+  // (64 per interpolated line and block; call after the stream is synchronised).
+  int interp_pairs(long long* pairs);
   int bins_per_block() const { return mixed_bins_per_block(R); }
   // Device bytes the runner holds now (its buffers only grow until release()).
   size_t bytes() const;

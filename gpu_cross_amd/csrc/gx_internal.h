@@ -200,7 +200,18 @@ struct MixedPlan {
                            // summed from moments per bin by k_near_eval
   unsigned long long* core_evals;  // kEvalSlots counters of core-pass evaluations (may be NULL):
                                    // the core pass adds the bins of U beyond each line's core
+  // interpolated lines of the region-I kernel (64-bin blocks): NULL switches them off
+  const float* interp_tab;         // [64][kInterpNodes] barycentric matrix, then the node offsets
+  unsigned long long* interp_cnt;  // kEvalSlots counters of interpolated (line, block) pairs (may be NULL)
 };
+
+// Chebyshev-Lobatto nodes of a 64-bin block for the interpolated lines of the region-I kernel.
+constexpr int kInterpNodes = 16;
+constexpr int kInterpTabNodes = 64 * kInterpNodes;   // offset of the node offsets in the table
+// The table: A[j][k] = L_k(s_j) with s_j = (j - 31.5) / 31.5 and the Lobatto nodes
+// s_k = cos(pi k / 15), then for each node -(its bin position - 32) (the anchored form's
+// offset sign); 64 x 16 + 16 floats.
+void interp_table(float* tab);
 
 // One request of a batched FP64 launch (launch_f64_batch): its constants, grid, line slice
 // [line_lo, line_lo + nlines) of the resident list, derived data (slice-local index) and sums.

@@ -721,6 +721,12 @@ int gx_server_result(gx_server* S, uint64_t id, double* out, int n, gx_timing* t
     t->total_ms = (float)std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count();
     t->evals = q->evals;
     t->lines = (long long)S->L->ls.n;
+    t->region1_ms = q->kernel_ms;
+    t->core_ms = 0.0f;
+    t->core_evals = 0;
+    t->far_ms = 0.0f;
+    t->far_evals = 0;
+    t->interp_pairs = 0;
   }
   S->reqs.erase(it);
   return GX_OK;

@@ -546,7 +546,14 @@ int gx_table_wait(gx_table* t, gx_timing* timing) {
   timing->core_evals = 0;
   timing->far_ms = 0.0f;
   timing->far_evals = 0;
+  timing->interp_pairs = 0;
   if (mixed_table(t)) {
+    for (Launch& l : t->launches) {
+      long long ip = 0;
+      const int irc = l.run.interp_pairs(&ip);
+      if (irc) return irc;
+      timing->interp_pairs += ip;
+    }
     if (t->graph_ran) {
       timing->region1_ms = t->eager_region1_ms;
       timing->core_ms = t->eager_core_ms;

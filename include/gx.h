@@ -86,6 +86,9 @@ typedef struct gx_timing {
     long long far_evals;    /* mixed precision: region-I line-bin pairs summed from moments
                                instead of one by one (the region-I kernel made
                                evals - core_evals - far_evals) */
+    long long interp_pairs; /* mixed precision: of the region-I kernel's pairs, those formed by
+                               interpolation from 16 node evaluations per line and 64-bin
+                               block (64 counted per line and block) */
 } gx_timing;
 
 /* BaseManager(ProfileType) + GpuManager ctor (GPUManager.cpp:5-17): bind

@@ -306,6 +306,39 @@ def test_voigt_mixed_near_leaves(voigt_lines, dense_lines, monkeypatch, R, T, P)
     assert rel_err(on, off) <= 2e-6
 
 
+@pytest.mark.parametrize("T,P,width", [(3000.0, 100.0, 0.01), (300.0, 6.3, 0.01), (1500.0, 100.0, 0.005),
+                                     (3000.0, 30.0, 0.0025)])
+def test_voigt_mixed_interpolated_lines(voigt_lines, dense_lines, monkeypatch, T, P, width):
+    """Interpolated lines (the default with 64-bin blocks; GX_INTERP=0 switches them off): a line
+    whose window covers a block, whose core misses it and whose Lorentz width is at least 1.5
+    half-blocks is evaluated at 16 Chebyshev-Lobatto nodes of the block and its sum interpolated
+    to the 64 bins.  Both forms against the FP64 oracle; the interpolated pairs are counted, and
+    the two forms agree far inside the tolerance."""
+    monkeypatch.setenv("GX_MIXED_R", "1")
+    ll, L = voigt_lines, dense_lines
+    Q = po.partition(ll.state_e, ll.state_g, T)
+    n = int(round(20.0 / width))
+    freq, d = make_grid(6040.0, 6060.0, n)
+    gam = L.gamma_at(P)
+    cut = 5.0
+    print(f"voigt mixed interpolated lines T={T} P={P} dnu={width} lines={L.n}", flush=True)
+    on, t_on = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                           cutoff=cut)
+    monkeypatch.setenv("GX_INTERP", "0")
+    off, t_off = run_manager(capi.VOIGT, L, freq, d, T, Q, 17.03, P, gamma=gam, precision=capi.PREC_MIXED,
+                             cutoff=cut)
+    c = po.consts(po.VOIGT, T, Q, d, 17.03, P, 500.0, cutoff=cut)
+    ref = po.cross_section(c, L.elow, L.nu, L.aif, L.gns, gam, L.nexp, freq, freq.size, 0, 4)
+    e_on, e_off = rel_err(on, ref), rel_err(off, ref)
+    direct = t_on.evals - t_on.far_evals - t_on.core_evals
+    print(f"  interpolated pairs {t_on.interp_pairs:.3e} of {direct:.3e} region-I pairs; "
+          f"max rel err {e_on:.3e} / {e_off:.3e}", flush=True)
+    assert e_on <= MIXED_TOL and e_off <= MIXED_TOL
+    assert t_on.evals == t_off.evals and t_on.far_evals == t_off.far_evals
+    assert t_off.interp_pairs == 0 and 0 < t_on.interp_pairs <= direct
+    assert rel_err(on, off) <= 1e-6
+
+
 @pytest.mark.parametrize("R,T,P", [("1", 3000.0, 1e-3), ("1", 1000.0, 100.0), ("2", 300.0, 10.0)])
 def test_voigt_mixed_far_field_small_blocks(voigt_lines, monkeypatch, R, T, P):
     """The far field with blocks of 64 and 128 bins (real and complex centres) against the
