@@ -98,11 +98,11 @@ struct MixedRunner {
   double* core_part = nullptr; // core pass: per-workgroup bin rows
   size_t core_cap = 0;
   int cus = 0;
-  // bins per lane (1, 2, 4 or 8): blocks of 64 R bins.  The manager and the server take 2: the
-  // far field then takes most lines of a block and the branch-free masked edge loop (R <= 2)
-  // the rest (one 1e8-line cross-section: 27.9 ms at R = 2, 27.7 at 1, 31.3 at 4); tables
-  // autotune it
-  int R = 2;
+  // bins per lane (1, 2, 4 or 8): blocks of 64 R bins.  The manager and the server take 1: the
+  // far field then takes most lines of a block, the branch-free masked edge loop (R <= 2) and
+  // the interpolated lines (R = 1) the rest (one 1e8-line cross-section: 25.3 ms at R = 1, 25.4
+  // at 2); tables autotune it
+  int R = 1;
   int items_per_slot = 4;      // work items per resident workgroup when splitting blocks
   int forced_nseg = 0;         // > 0: fixed segments per block
   bool persistent = true;      // false: one workgroup per work item, so the hardware interleaves

@@ -87,7 +87,7 @@ struct gx_table {
   float eager_region1_ms = 0, eager_core_ms = 0, eager_far_ms = 0;
   unsigned long long eager_far_evals = 0;
   unsigned long long* core_evals = nullptr;   // kEvalSlots counters of the core pass
-  int R = 4, ips = 4;
+  int R = 1, ips = 4;          // launch shape without the autotuner (the one it picks for tables)
 };
 
 namespace {
