@@ -267,12 +267,37 @@ __device__ __forceinline__ unsigned long long prep_mixed_point(const PointConsts
     const double half = (15.0 - y) * t.alpha;
     const double lo = floor((nu_if - half - gv.start_nu) * inv_dfreq) - 1.0;
     const double hi = ceil((nu_if + half - gv.start_nu) * inv_dfreq) + 1.0;
-    int a0 = (int)fmax(lo, (double)ib), z = (int)fmin(hi, (double)ie);
-    while (a0 < z && !in_core(nu_if, g, y, gv.freq, a0)) ++a0;
-    while (z > a0 && !in_core(nu_if, g, y, gv.freq, z - 1)) --z;
-    if (z > a0) {
-      k0 = a0;
-      k1 = z;
+    const int a0 = (int)fmax(lo, (double)ib), z0 = (int)fmin(hi, (double)ie);
+    if (a0 < z0) {
+      // The core is a contiguous run of the bracket (|nu - f_j| grows away from nu), one or two
+      // bins inside each end: the first three bins from each end are tested together (their
+      // grid loads in flight at once), the rest one by one only if those miss.
+      const int l1 = min(a0 + 1, z0 - 1), l2 = min(a0 + 2, z0 - 1);
+      const int r1 = max(z0 - 2, a0), r2 = max(z0 - 3, a0);
+      const double* fq = gv.freq;
+      const double fl0 = fq[a0], fl1 = fq[l1], fl2 = fq[l2], fr0 = fq[z0 - 1], fr1 = fq[r1], fr2 = fq[r2];
+      auto inside = [&](double f) { return fabs(nu_if - f) * g + y < 15.0; };
+      const bool tl0 = inside(fl0), tl1 = inside(fl1), tl2 = inside(fl2);
+      const bool tr0 = inside(fr0), tr1 = inside(fr1), tr2 = inside(fr2);
+      int a, z;
+      if (tl0) a = a0;
+      else if (a0 + 1 < z0 && tl1) a = a0 + 1;
+      else if (a0 + 2 < z0 && tl2) a = a0 + 2;
+      else {
+        a = min(a0 + 3, z0);
+        while (a < z0 && !in_core(nu_if, g, y, fq, a)) ++a;
+      }
+      if (tr0) z = z0;
+      else if (z0 - 2 >= a0 && tr1) z = z0 - 1;
+      else if (z0 - 3 >= a0 && tr2) z = z0 - 2;
+      else {
+        z = max(z0 - 3, a0);
+        while (z > a0 && !in_core(nu_if, g, y, fq, z - 1)) --z;
+      }
+      if (z > a) {
+        k0 = a;
+        k1 = z;
+      }
     }
   }
   out.core[(size_t)p * out.stride + i] = make_int2(k0, k1);
