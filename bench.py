@@ -216,7 +216,7 @@ def bench_table(args, wl):
         "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV; region I of near ranges)", "ms": core_ms, "evals": core_evals,
                       "evals_per_s": core_evals / (core_ms * 1e-3) if core_ms > 0 else None},
         "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, "
-                                 "k_near_eval, k_far_edges (FP32 leaf sums, FP64 tree and series; near leaves summed "
+                                 "k_near_eval, k_far_edges (FP32 leaf sums and records, FP64 tree shifts and series; near leaves summed "
                                  "per bin)",
                       "ms": far_ms, "evals": far_evals, "share": far_evals / max(evals, 1)},
         "intensity_kernels_ms": eager_int,

@@ -385,7 +385,7 @@ void MixedRunner::release() {
 
 size_t MixedRunner::bytes() const {
   return (size_t)task_cap * sizeof(MixedTask) + partial_cap * sizeof(float2) + starts_cap * sizeof(int) +
-         core_cap * sizeof(double) + far_geom_cap * sizeof(double4) + far_rec_cap * sizeof(double) +
+         core_cap * sizeof(double) + far_geom_cap * sizeof(double4) + far_rec_cap * sizeof(FarVal) +
          far_blk_cap * (2 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + (size_t)far_kind_cap * sizeof(int) +
          2 * sizeof(int) * (size_t)far_first_cap + near_u_cap * sizeof(int2) + near_rng_cap * sizeof(int4);
 }
@@ -617,7 +617,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
         hipFree(far_rec);
         far_rec = nullptr;
         far_rec_cap = 0;
-        if ((e = hipMalloc(&far_rec, sizeof(double) * need_rec))) return hip_fail(e, "hipMalloc(far moments)");
+        if ((e = hipMalloc(&far_rec, sizeof(FarVal) * need_rec))) return hip_fail(e, "hipMalloc(far moments)");
         far_rec_cap = need_rec;
       }
       if (need_blk > far_blk_cap) {
@@ -695,7 +695,7 @@ int MixedRunner::prepare(const MixedTask* tasks, int ntasks, int nblocks, const 
 // Bytes of the far-field buffers for ntasks points over nlines lines and nblocks blocks.
 size_t far_field_bytes(int ntasks, long long nlines, int nblocks) {
   const long long clusters = 2 * ((nlines + kFarLeaf - 1) / kFarLeaf) + 64;
-  return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * (kFarRec * sizeof(double) + sizeof(int2))) +
+  return (size_t)clusters * (sizeof(double4) + (size_t)ntasks * (kFarRec * sizeof(FarVal) + sizeof(int2))) +
          (size_t)ntasks * nblocks * (3 * sizeof(int4) + sizeof(int) * (kFarList + 1)) + sizeof(int) * (size_t)ntasks +
          2 * sizeof(int) * ((size_t)nblocks * 64 + 1);    // per-bin line bounds (callers count 64-bin blocks)
 }

@@ -111,8 +111,8 @@ struct MixedRunner {
   bool far_on = true;
   double4* far_geom = nullptr;
   size_t far_geom_cap = 0;     // clusters
-  double* far_rec = nullptr;
-  size_t far_rec_cap = 0;      // doubles
+  FarVal* far_rec = nullptr;
+  size_t far_rec_cap = 0;      // values
   int4* far_rng = nullptr;
   int4* edge_rng = nullptr;
   int* far_list = nullptr;

@@ -33,7 +33,7 @@
 // core reaches the block (so every pair is a region-I pair of the FP64 path too); the
 // region-I kernel skips those lines and k_far_eval adds their sums in FP64.
 //
-// Cluster record (kFarRec doubles): radius, max 1/g, min y, lowest and highest gamma,
+// Cluster record (kFarRec FP32 values): radius, max 1/g, min y, lowest and highest gamma,
 // gamma_c (0 for real centres), then Q_1..Q_24 (real centres) or (Re, Im) of M_0..M_23.
 // Each task's records are stored field-major (FarRec): field f of cluster k at
 // f * nclus + k, so threads on consecutive clusters read and write consecutive doubles.
@@ -68,7 +68,7 @@ struct FarRec {
   size_t stride;   // clusters of the task (the distance between fields)
   __device__ __forceinline__ T& operator[](int f) const { return p[(size_t)f * stride]; }
 };
-using FarRecC = FarRec<const double>;
+using FarRecC = FarRec<const FarVal>;
 
 // Cluster geometry, for every level: centre = midpoint of its lines' nu range.
 __global__ void k_far_geom(MixedPlan P, double4* __restrict__ geom) {
@@ -169,7 +169,7 @@ constexpr int kLeafTasks = 8;
 // the powers and the wave sums run in FP32: a moment's rounding error is then ~1e-7 of the
 // sum of its terms' magnitudes, and the series weighs moment k by at most eta^(k+1), so the
 // far sums keep ~1e-7 relative accuracy (the weights are 0.28 S, S/alpha being FP32 in the
-// mixed path already).  The records stay FP64 for the tree and the plans.
+// mixed path already).  The records are stored in FP32 (FarVal) and every reader computes in FP64.
 __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int l, bool act, double nu, double c,
                                             int lane, FarRec<double> rec) {
   {
@@ -191,11 +191,11 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
       glo = (double)wave_extreme_f32<false>(act ? f32_bound<false>(im) : INFINITY);
       ghi = (double)wave_extreme_f32<true>(act ? f32_bound<true>(im) : 0.0f);
     }
-    const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
+    const double gc = cplx ? (double)(float)(0.5 * (glo + ghi)) : 0.0;   // stored as FP32
     const double wim = im - gc;                                  // Im (zeta - centre)
     const double r2 =
         (double)wave_extreme_f32<true>(act ? f32_bound<true>(fmax(dp * dp, dm * dm) + wim * wim) : 0.0f);
-    double r = sqrt(r2);
+    double r = (double)f32_bound<true>(sqrt(r2));               // stored as FP32, rounded up
     if (!(r > 0.0)) r = 1.0;
     const double ir = 1.0 / r;
     const float kap = act ? (float)(kKappa * (double)amp * al) : 0.0f;   // amp = S g
@@ -322,10 +322,10 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
     }
     __syncthreads();
     const int nf = cplx ? kFarRec : kFarHead + kFarP;
-    double* dst = P.far_rec + (size_t)task * ns * kFarRec + leaf0;
+    FarVal* dst = P.far_rec + (size_t)task * ns * kFarRec + leaf0;
     for (int idx = threadIdx.x; idx < nf * 4; idx += 256) {
       const int f = idx >> 2, lf = idx & 3;
-      if (lf < nleaf) dst[(size_t)f * ns + lf] = s_rec[lf][f];
+      if (lf < nleaf) dst[(size_t)f * ns + lf] = (FarVal)s_rec[lf][f];   // the head is FP32 already
     }
     __syncthreads();
   }
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
 __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.far_n[lev]) return;
-  double* const tb = P.far_rec + (size_t)blockIdx.y * P.far_nclus * kFarRec;   // the task's records
+  FarVal* const tb = P.far_rec + (size_t)blockIdx.y * P.far_nclus * kFarRec;   // the task's records
   const size_t ns = (size_t)P.far_nclus;
   const bool cplx = P.far_kind[blockIdx.y] != 0;
   const int self = P.far_off[lev] + i;
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     glo = fmin(glo, cr[3]);
     ghi = fmax(ghi, cr[4]);
   }
-  const double gc = cplx ? 0.5 * (glo + ghi) : 0.0;
+  const double gc = cplx ? (double)(float)(0.5 * (glo + ghi)) : 0.0;   // the stored FP32 value
   double r = 0.0;
   for (int h = 0; h < nch; ++h) {
     const int ch = P.far_off[lev - 1] + 2 * i + h;
@@ -368,6 +368,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     const double dx = P.far_geom[ch].x - c, dy = cr[5] - gc;
     r = fmax(r, (cplx ? sqrt(dx * dx + dy * dy) : fabs(dx)) + cr[0]);
   }
+  r = (double)f32_bound<true>(r);                // the stored FP32 value, rounded up
   if (!(r > 0.0)) r = 1.0;
   if (P.near) {                                  // hull of the children's near ranges
     int2 u = make_int2(0, 0);
@@ -378,13 +379,13 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     }
     P.near_u[(size_t)blockIdx.y * ns + self] = u;
   }
-  const FarRec<double> rec{tb + self, ns};
-  rec[0] = r;
-  rec[1] = amax;
-  rec[2] = ymin;
-  rec[3] = glo;
-  rec[4] = ghi;
-  rec[5] = gc;
+  const FarRec<FarVal> rec{tb + self, ns};      // every head field is an FP32 value already
+  rec[0] = (FarVal)r;
+  rec[1] = (FarVal)amax;
+  rec[2] = (FarVal)ymin;
+  rec[3] = (FarVal)glo;
+  rec[4] = (FarVal)ghi;
+  rec[5] = (FarVal)gc;
   if (!cplx) {
     double q[kFarP];
 #pragma unroll
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
       }
     }
 #pragma unroll
-    for (int k = 0; k < kFarP; ++k) rec[kFarHead + k] = q[k];
+    for (int k = 0; k < kFarP; ++k) rec[kFarHead + k] = (FarVal)q[k];
   } else {
     double qr[kFarP], qi[kFarP];
 #pragma unroll
@@ -451,8 +452,8 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     }
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) {
-      rec[kFarHead + 2 * k] = qr[k];
-      rec[kFarHead + 2 * k + 1] = qi[k];
+      rec[kFarHead + 2 * k] = (FarVal)qr[k];
+      rec[kFarHead + 2 * k + 1] = (FarVal)qi[k];
     }
   }
 }
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(256) void k_far_plan(MixedPlan P) {
     const int L1 = first_above<false>(P, rg.x, rg.y, w1 - 1);
     const int L2 = first_above<true>(P, rg.x, rg.y, w0);
     const int leafL = (L1 + kFarLeaf - 1) / kFarLeaf, leafR = L2 / kFarLeaf;
-    const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+    const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
     int* list = P.far_list + slot * kFarList;
     const bool cplx = P.far_kind[task] != 0;
     auto usable = [&](int lev, int idx) -> bool {
@@ -659,7 +660,7 @@ __global__ __launch_bounds__(256) void k_edge_plan(MixedPlan P) {
     const int Ly = first_above<true>(P, rg.x, rg.y, w1 - 1);     // ib >= w1 from here
     const int em0 = Lx, em1 = min(L1, L2) / kFarLeaf * kFarLeaf;
     const int ep0 = (max(L1, L2) + kFarLeaf - 1) / kFarLeaf * kFarLeaf, ep1 = Ly;
-    const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+    const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
     const bool cplx = P.far_kind[task] != 0;
     auto leaves_usable = [&](int a, int z) -> bool {       // every lane calls it together
       bool ok = true;
@@ -713,7 +714,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const GridView gv = P.gv;
   const int w0 = gv.start_idx + b * 64 * R;
   const int* list = P.far_list + slot_b * kFarList;
-  const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const size_t ns = (size_t)P.far_nclus;
   const int slot = lane / BL, bl = lane % BL;
   double f[Q], acc[Q];
@@ -832,7 +833,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (nr.x >= nr.y && nr.z >= nr.w) return;
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
-  const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const int2* nu_t = P.near_u + (size_t)task * P.far_nclus;
   const size_t ns = (size_t)P.far_nclus;
   const int slot = lane / BL, bl = lane % BL;
@@ -945,7 +946,7 @@ __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedT
 
 // Region I of the lines [a, z) (every one contributing) at bin f: leaves of the range
 // from the largest aligned clusters that serve f, partial leaves line by line.
-__device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, bool cplx,
+__device__ __forceinline__ double range_sum(const MixedPlan& P, const MixedTask& tk, const FarVal* __restrict__ recs, bool cplx,
                             int a, int z, double f, int task, int jb) {
   if (z <= a) return 0.0;
   const int A = (a + kFarLeaf - 1) / kFarLeaf, B = z / kFarLeaf;
@@ -1125,7 +1126,7 @@ __device__ __forceinline__ void block_scan(double (&v)[N], double* s_tot) {
 // else lo is the same and hi rises.  From one shifted series when it meets the truncation
 // bound at every bin of the block, else through the tree bin by bin.
 template <int NW, bool CPLX, bool SUFFIX>
-__device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask& tk, const double* __restrict__ recs, int lo,
+__device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask& tk, const FarVal* __restrict__ recs, int lo,
                              int hi, double f, double f_lo, double f_hi, int* s_cut, double* s_tot, double* s_red,
                              int task, int jb) {
   constexpr int NB = 64 * NW;
@@ -1194,7 +1195,7 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   if ((P.far_kind[task] != 0) != CPLX) return;                 // the other kind's launch
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
-  const double* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
+  const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const int w0 = gv.start_idx + b * NB;
   const int j = w0 + threadIdx.x;
   const double f = gv.freq[j], f_lo = gv.freq[w0], f_hi = gv.freq[w0 + NB - 1];

@@ -131,6 +131,10 @@ constexpr double kFarEta = 0.5;      // a cluster of radius r serves bins at dis
 constexpr int kFarLeaf = 64;         // lines per leaf
 constexpr int kFarHead = 6;          // radius, max 1/g, min y, lowest and highest gamma, gamma of the centre
 constexpr int kFarRec = kFarHead + 2 * kFarP;   // head, then P real or P complex moments
+// Cluster records are stored in FP32: the bound fields (radius, largest Doppler width, lowest y,
+// Lorentz-width range) rounded outward, the complex centre's gamma and the moments to nearest;
+// every kernel computes in FP64 from them (the tree shifts, the series, the plans).
+using FarVal = float;
 constexpr double kFarTol = 5e-7;     // bound on the relative truncation error of every far-field sum
 constexpr double kFarCplxGamma = 0.05;  // points whose sampled Lorentz widths are all >= this (cm^-1)
                                         // take complex-centred moments
@@ -181,7 +185,7 @@ struct MixedPlan {
   int far_n[kFarMaxLevels];     // clusters of each level
   int far_nclus;           // clusters of all levels
   double4* far_geom;       // [far_nclus]: centre, lowest nu, highest nu (shared by the tasks)
-  double* far_rec;         // [ntasks][kFarRec][far_nclus]: head and moments of each cluster, field-major
+  FarVal* far_rec;         // [ntasks][kFarRec][far_nclus]: head and moments of each cluster, field-major
   int* far_kind;           // [ntasks]: 0 real-centred moments, 1 complex-centred
   int4* far_rng;           // [ntasks][nblocks]: far line ranges [x, y) and [z, w) of each block
   int4* edge_rng;          // [ntasks][nblocks]: window-end lines from moments: [x, y) end inside the
