@@ -36,7 +36,7 @@
 // Cluster record (kFarRec FP32 values): radius, max 1/g, min y, lowest and highest gamma,
 // gamma_c (0 for real centres), then Q_1..Q_24 (real centres) or (Re, Im) of M_0..M_23.
 // Each task's records are stored field-major (FarRec): field f of cluster k at
-// f * nclus + k, so threads on consecutive clusters read and write consecutive doubles.
+// f * nclus + k, so threads on consecutive clusters read and write consecutive values.
 #include <hip/hip_runtime.h>
 
 #include "gx_internal.h"
@@ -1201,18 +1201,35 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   const double f = gv.freq[j], f_lo = gv.freq[w0], f_hi = gv.freq[w0 + NB - 1];
   unsigned long long pairs = 0;
   double acc = 0.0;
+  // The partial leaf of a range at bin j: its lines in the range one by one, or, when they are
+  // more than half of it, the leaf's series (it joins the whole leaves; every leaf the range
+  // touches serves the block) less its other lines one by one.  The threads' first (last) whole
+  // leaf still rises with j, as leaves_sum needs: the partial part shrinks (grows) along j.
   if (er.x < er.y) {                                           // er.y is a leaf boundary
     const int s0 = first_above<false>(P, er.x, er.y, j);       // ie > j from here
-    const int A = (s0 + kFarLeaf - 1) / kFarLeaf;
+    int A = (s0 + kFarLeaf - 1) / kFarLeaf;
     pairs += (unsigned long long)(er.y - s0);
-    acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
+    const int part = A * kFarLeaf - s0;
+    if (2 * part > kFarLeaf) {
+      acc -= region1_lines(P, tk, (A - 1) * kFarLeaf, s0, f);
+      --A;
+    } else {
+      acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
+    }
     acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   if (er.z < er.w) {                                           // er.z is a leaf boundary
     const int p1 = first_above<true>(P, er.z, er.w, j);        // ib <= j before here
-    const int D = p1 / kFarLeaf;
+    int D = p1 / kFarLeaf;
     pairs += (unsigned long long)(p1 - er.z);
-    acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
+    const int leaf_end = min((D + 1) * kFarLeaf, P.nlines);
+    const int part = p1 - D * kFarLeaf;
+    if (2 * part > leaf_end - D * kFarLeaf) {
+      acc -= region1_lines(P, tk, p1, leaf_end, f);
+      ++D;
+    } else {
+      acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
+    }
     acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   tk.out[j] += acc;
