@@ -343,12 +343,14 @@ __global__ __launch_bounds__(256) void k_far_leaf(MixedPlan P) {
 // and scale b = r_child / r, Q_k = sum_m C(k, m) a^(k-m) b^m Q_m(child).  Complex centres:
 // the same with a complex shift, M_k = sum_m C(k, m) a^(k-m) b^m M_m(child), formed by
 // repeated Taylor-shift passes v_k += a v_(k-1) over v_m = b^m M_m(child).
+template <bool CPLX>
 __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.far_n[lev]) return;
+  if ((P.far_kind[blockIdx.y] != 0) != CPLX) return;         // the other kind's launch
   FarVal* const tb = P.far_rec + (size_t)blockIdx.y * P.far_nclus * kFarRec;   // the task's records
   const size_t ns = (size_t)P.far_nclus;
-  const bool cplx = P.far_kind[blockIdx.y] != 0;
+  constexpr bool cplx = CPLX;
   const int self = P.far_off[lev] + i;
   const double c = P.far_geom[self].x;
   const int nch = min(2, P.far_n[lev - 1] - 2 * i);
@@ -386,7 +388,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
   rec[3] = (FarVal)glo;
   rec[4] = (FarVal)ghi;
   rec[5] = (FarVal)gc;
-  if (!cplx) {
+  if constexpr (!cplx) {
     double q[kFarP];
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) q[k] = 0.0;
@@ -394,27 +396,22 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
       const int ch = P.far_off[lev - 1] + 2 * i + h;
       const FarRecC cr{tb + ch, ns};
       const double a = (P.far_geom[ch].x - c) / r, b = cr[0] / r;
-      double pa[kFarP], qb[kFarP];                             // a^j, b^m Q_m
-      pa[0] = 1.0;
+      // Q_k (k = 1..P) += sum_{m=1..k} C(k, m) a^(k-m) b^m Q_m: v_m = b^m Q_m (v_0 = Q_0 = 0 for a
+      // real centre), then P in-place Taylor-shift passes v_k += a v_(k-1)
+      double v[kFarP + 1];
+      v[0] = 0.0;
       double bm = b;
 #pragma unroll
-      for (int j = 0; j < kFarP; ++j) {
-        if (j) pa[j] = pa[j - 1] * a;
-        qb[j] = bm * cr[kFarHead + j];
+      for (int m = 1; m <= kFarP; ++m) {
+        v[m] = bm * cr[kFarHead + m - 1];
         bm *= b;
       }
-      // Q_k (k = 1..P) += sum_{m=1..k} C(k, m) a^(k-m) b^m Q_m
 #pragma unroll
-      for (int k = 1; k <= kFarP; ++k) {
-        double cb = k;                                         // C(k, 1)
-        double acc = 0.0;
+      for (int j = 1; j <= kFarP; ++j)
 #pragma unroll
-        for (int m = 1; m <= k; ++m) {
-          acc += cb * pa[k - m] * qb[m - 1];
-          cb = cb * (double)(k - m) / (double)(m + 1);
-        }
-        q[k - 1] += acc;
-      }
+        for (int k = kFarP; k >= j; --k) v[k] = v[k] + a * v[k - 1];
+#pragma unroll
+      for (int k = 1; k <= kFarP; ++k) q[k - 1] += v[k];
     }
 #pragma unroll
     for (int k = 0; k < kFarP; ++k) rec[kFarHead + k] = (FarVal)q[k];
@@ -1248,8 +1245,11 @@ hipError_t launch_far_setup(const MixedPlan& plan, hipStream_t s) {
   k_far_reset<<<1, 1, 0, s>>>(plan.far_evals);
   k_far_kind<<<plan.ntasks, 256, 0, s>>>(plan);
   k_far_leaf<<<dim3(((plan.far_n[0] + 3) / 4 + 7) / 8 * 8, (plan.ntasks + kLeafTasks - 1) / kLeafTasks), 256, 0, s>>>(plan);
-  for (int lev = 1; lev < plan.far_nlev; ++lev)
-    k_far_up<<<dim3((plan.far_n[lev] + 255) / 256, plan.ntasks), 256, 0, s>>>(plan, lev);
+  for (int lev = 1; lev < plan.far_nlev; ++lev) {   // one launch per moment kind: the complex
+    const dim3 g((plan.far_n[lev] + 255) / 256, plan.ntasks);   // shifts hold far more registers
+    k_far_up<false><<<g, 256, 0, s>>>(plan, lev);
+    k_far_up<true><<<g, 256, 0, s>>>(plan, lev);
+  }
   k_bin_first<<<(plan.gv.points + 1 + 255) / 256, 256, 0, s>>>(plan);
   k_far_plan<<<dim3((plan.nblocks + 255) / 256, plan.ntasks), 256, 0, s>>>(plan);
   k_edge_plan<<<dim3((plan.nblocks + 3) / 4, plan.ntasks), 256, 0, s>>>(plan);
