@@ -118,11 +118,22 @@ struct MixedTask {
   double* out;
 };
 
+// A load through the global address space: pointers read from memory (a task table) are
+// generic to the compiler, and its flat loads also hold the LDS wait counter.
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ int2 ldg(const int2* p) {
+  const unsigned long long v = *(const __attribute__((address_space(1))) unsigned long long*)p;
+  return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+}
+
 // y = gamma_T / alpha of line l of a mixed task: ((gamma pressure_scale) (T_ref/T)^n) g, each
 // product rounded as the FP64 path rounds it (k_line_prep), so the bits are that path's.
 __device__ __forceinline__ double mixed_y(const MixedTask& tk, int l, double g) {
 #pragma clang fp contract(off)
-  return ((tk.gamma[l] * tk.pscale) * tk.ts[l]) * g;
+  return ((ldg(tk.gamma + l) * tk.pscale) * ldg(tk.ts + l)) * g;
 }
 
 // Far field of region I: moments per cluster, leaf size, list length, levels.

@@ -721,16 +721,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
 #pragma clang fp contract(off)
         const double v0 = P.nu[i];
         const double nu = v0 == 0.0 ? 1e-6 : v0;
-        const double g = tk.g[i], y = mixed_y(tk, i, g);            // the FP64 path's bits
+        const double g = ldg(tk.g + i), y = mixed_y(tk, i, g);      // the FP64 path's bits
         const double cj = rint((nu - gv.start_nu) * inv_dfreq);     // nearest bin (r below is consistent)
         const double fc = gv.start_nu + cj * gv.dfreq;
         const int ib = P.ib[i], ie = P.ie[i];
-        int2 core = tk.core[i];                 // bins left to the core pass: the W4 core, or the
+        int2 core = ldg(tk.core + i);           // bins left to the core pass: the W4 core, or the
         if (P.near) {                           // near range of the line's leaf when it has one
           const int2 u = P.near_u[(size_t)task * P.far_nclus + i / kFarLeaf];
           if (u.y > u.x) core = u;
         }
-        const float amp = tk.amp[i];
+        const float amp = ldg(tk.amp + i);
         // Power-of-two rescaling x' = sigma x keeps x'^4 far from FP32 overflow for lines
         // whose window spans huge x (tiny nu => huge 1/alpha); region I is invariant under
         // (A1, A2, R, P2) -> (A1 s^4, A2 s^2, R s^2, P2 s^4).  sigma = 1 for ordinary lines.
@@ -812,10 +812,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R <= 4 ?
       const unsigned long long ma = __ballot(fa), mx = __ballot(fx), mea = __ballot(ea), me = __ballot(ed);
       const unsigned long long mi = __ballot(fi);
       const int na = __popcll(ma), nx = __popcll(mx), nea = __popcll(mea), ne = __popcll(me), ni = __popcll(mi);
-      const unsigned long long mine = fa ? ma : (fx ? mx : (ea ? mea : (ed ? me : mi)));
-      const int pos = (fx ? na : 0) + (ea ? na + nx : 0) + (ed ? na + nx + nea : 0) +
-                      (fi ? na + nx + nea + ne : 0) +
-                      __builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+      // a chunk of one class (the common case: all its lines fast or all interpolated) keeps the
+      // lines' order among themselves; others are sorted by class
+      const unsigned long long mall = ma | mx | mea | me | mi;
+      int pos;
+      if (mall == ma || mall == mi || mall == mx || mall == mea || mall == me) {
+        pos = __builtin_amdgcn_mbcnt_hi((unsigned)(mall >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mall, 0u));
+      } else {
+        const unsigned long long mine = fa ? ma : (fx ? mx : (ea ? mea : (ed ? me : mi)));
+        pos = (fx ? na : 0) + (ea ? na + nx : 0) + (ed ? na + nx + nea : 0) + (fi ? na + nx + nea + ne : 0) +
+              __builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+      }
       wave_lds_handover();                         // the previous chunk's reads are done
       if (fa || fx || ea || ed || fi) {
         my_l[pos][0] = rec[0];
@@ -1112,10 +1119,10 @@ __device__ __forceinline__ void core_chunks(const MixedPlan& P, const MixedTask&
         const double v0 = P.nu[l];
         L[r].nu = v0 == 0.0 ? 1e-6 : v0;
         if (r == 0) cb = centre_bin(L[r].nu, gv);
-        L[r].g = tk.g[l];
+        L[r].g = ldg(tk.g + l);
         L[r].y = mixed_y(tk, l, L[r].g);
-        L[r].amp = tk.amp[l];
-        int2 core = tk.core[l];
+        L[r].amp = ldg(tk.amp + l);
+        int2 core = ldg(tk.core + l);
         if (P.near) {                            // the near range of the line's leaf (it holds the
           const int2 u = P.near_u[(size_t)blockIdx.y * P.far_nclus + l / kFarLeaf];   // line's core)
           if (u.y > u.x) core = u;
