@@ -117,8 +117,26 @@ __device__ __forceinline__ float dpp_f32(float v) {   // every source lane valid
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
+// gfx950 lane swaps: permlane32_swap exchanges the upper 32 lanes of a with the lower 32 lanes
+// of b, permlane16_swap the odd 16-lane rows of a with the even rows of b.  After the swap, in
+// every lane one of (a, b) holds the lane's own value and the other the value of its partner at
+// lane distance D (a's in the lower half or the even rows, b's in the others).
+template <int D>
+__device__ __forceinline__ void lane_swap(float& a, float& b) {
+  static_assert(D == 32 || D == 16, "lane distance of a swap");
+  if constexpr (D == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+  }
+}
+
 // Maximum (MAX) or minimum over the wave, in every lane: quad swaps and rotations by 4 and
-// 8 inside each row of 16 lanes, then across rows.
+// 8 inside each row of 16 lanes, then across rows by the lane swaps.
 template <bool MAX>
 __device__ __forceinline__ float wave_extreme_f32(float v) {
   auto pick = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
@@ -126,38 +144,57 @@ __device__ __forceinline__ float wave_extreme_f32(float v) {
   v = pick(v, dpp_f32<0x4E>(v));          // quad_perm [2, 3, 0, 1]
   v = pick(v, dpp_f32<0x124>(v));         // row_ror 4
   v = pick(v, dpp_f32<0x128>(v));         // row_ror 8
-  v = pick(v, __shfl_xor(v, 16));
-  return pick(v, __shfl_xor(v, 32));
+  float a = v, b = v;
+  lane_swap<16>(a, b);
+  v = pick(a, b);
+  a = v;
+  b = v;
+  lane_swap<32>(a, b);
+  return pick(a, b);
 }
 
 // Sums over the 64 lanes of N values per lane (N a multiple of 8), by halving: at lane
 // distance 32, 16 and 8 each lane keeps half of its values and adds the partner's copy of
-// that half, then the N/8 values left are summed over the 8 lanes (distances 4, 2, 1).
-// Afterwards every lane holds the totals of values first(lane) + [0, N/8), first(lane) =
-// N/2 bit5 + N/4 bit4 + N/8 bit3 of its lane index.  Fixed order (a + b = b + a in IEEE):
-// the totals do not depend on scheduling.
-template <int D, int H, int N, class T>
-__device__ __forceinline__ void wave_halve(T (&v)[N], int lane) {
+// that half (one lane swap of the value pair at 32 and 16, a row rotation by 8 inside each
+// row of 16), then the N/8 values left are summed over the 8 lanes (row rotation by 4, quad
+// swaps by 2 and 1) into lane 8m + 4 of each group of eight.  There lane 8m + 4 holds the
+// totals of values first(lane) + [0, N/8), first(lane) = N/2 bit5 + N/4 bit4 + N/8 bit3 of its
+// lane index.  Fixed order (a + b = b + a in IEEE): the totals do not depend on scheduling.
+// Every lane of the wave must be active.
+template <int D, int H, int N>
+__device__ __forceinline__ void wave_halve(float (&v)[N], int lane) {
 #pragma clang fp contract(off)
-  const bool up = (lane & D) != 0;
+  if constexpr (D == 8) {
+    const bool up = (lane & D) != 0;
 #pragma unroll
-  for (int i = 0; i < H; ++i) {
-    const T keep = up ? v[H + i] : v[i];
-    const T send = up ? v[i] : v[H + i];
-    v[i] = keep + __shfl_xor(send, D);
+    for (int i = 0; i < H; ++i) {
+      const float keep = up ? v[H + i] : v[i];
+      const float send = up ? v[i] : v[H + i];
+      v[i] = keep + dpp_f32<0x128>(send);     // row_ror 8: the lane at distance 8 in the row
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      lane_swap<D>(v[i], v[H + i]);
+      v[i] = v[i] + v[H + i];
+    }
   }
 }
 
-template <int N, class T>
-__device__ __forceinline__ void wave_sums(T (&v)[N], int lane) {
+template <int N>
+__device__ __forceinline__ void wave_sums(float (&v)[N], int lane) {
 #pragma clang fp contract(off)
   wave_halve<32, N / 2>(v, lane);
   wave_halve<16, N / 4>(v, lane);
   wave_halve<8, N / 8>(v, lane);
+  // row_ror 4 reads lane l - 4 of the row, so lanes 8m + 4 .. 8m + 7 hold the pair sums of
+  // their group; quad swaps by 2 and 1 inside the quad 8m + 4 .. 8m + 7 finish the group
 #pragma unroll
-  for (int d = 4; d >= 1; d >>= 1)
-#pragma unroll
-    for (int i = 0; i < N / 8; ++i) v[i] += __shfl_xor(v[i], d);
+  for (int i = 0; i < N / 8; ++i) {
+    v[i] = v[i] + dpp_f32<0x124>(v[i]);
+    v[i] = v[i] + dpp_f32<0x4E>(v[i]);
+    v[i] = v[i] + dpp_f32<0xB1>(v[i]);
+  }
 }
 
 // Points per wave of the leaf kernel: each wave reads its lines' nu once for them.
@@ -214,14 +251,15 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
     const int first = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
     if (!cplx) {
       // Im w^k of each pole by the real recurrence z_k = 2 Re(w) z_(k-1) - |w|^2 z_(k-2)
-      // (w and its conjugate are the roots of t^2 - 2 Re(w) t + |w|^2), z_0 = 0, z_1 = Im w
+      // (w and its conjugate are the roots of t^2 - 2 Re(w) t + |w|^2), z_0 = 0, z_1 = Im w,
+      // each scaled by the line weight kap
       float v[kFarP];
       const float tp = 2.0f * wrp, np = wrp * wrp + wi * wi;
       const float tm = 2.0f * wrm, nm = wrm * wrm + wi * wi;
-      float p1 = wi, p0 = 0.0f, m1 = wi, m0 = 0.0f;
+      float p1 = kap * wi, p0 = 0.0f, m1 = p1, m0 = 0.0f;        // the weight kap carried along
 #pragma unroll
       for (int k = 0; k < kFarP; ++k) {
-        v[k] = kap * (p1 + m1);
+        v[k] = p1 + m1;
         const float p2 = tp * p1 - np * p0, m2 = tm * m1 - nm * m0;
         p0 = p1;
         p1 = p2;
@@ -229,17 +267,17 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
         m1 = m2;
       }
       wave_sums(v, lane);
-      if ((lane & 7) == 0) {
+      if ((lane & 7) == 4) {
 #pragma unroll
         for (int t = 0; t < kFarP / 8; ++t) rec[kFarHead + first * (kFarP / 8) + t] = (double)v[t];
       }
     } else {
       float v[2 * kFarP];                                        // (Re, Im) of M_0..M_(P-1)
-      float pr = 1.0f, pi = 0.0f, mr = 1.0f, mi = 0.0f;          // w+^0, w-^0
+      float pr = kap, pi = 0.0f, mr = kap, mi = 0.0f;            // kap w+^0, kap w-^0
 #pragma unroll
       for (int k = 0; k < kFarP; ++k) {
-        v[2 * k] = kap * (pr + mr);
-        v[2 * k + 1] = kap * (pi + mi);
+        v[2 * k] = pr + mr;
+        v[2 * k + 1] = pi + mi;
         const float nr = pr * wrp - pi * wi, ni = pr * wi + pi * wrp;
         const float n2r = mr * wrm - mi * wi, n2i = mr * wi + mi * wrm;
         pr = nr;
@@ -248,7 +286,7 @@ __device__ __forceinline__ void leaf_record(const MixedTask& tk, bool cplx, int 
         mi = n2i;
       }
       wave_sums(v, lane);
-      if ((lane & 7) == 0) {
+      if ((lane & 7) == 4) {
 #pragma unroll
         for (int t = 0; t < kFarP / 4; ++t) rec[kFarHead + first * (kFarP / 4) + t] = (double)v[t];
       }
@@ -906,10 +944,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (lane == 0 && pairs) atomicAdd(P.far_evals, pairs);
 }
 
-// Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
-// kernel's: c y (u + 1) / (u^2 + 2 y^2), u = x^2 + y^2 - 1/2, scaled by amp = S g).
+// Region I of one line at bin f, scaled by amp = S g: c y (u + 1) / (u^2 + 2 y^2) with
+// u = x^2 + y^2 - 1/2 (the region-I kernel's form).  x is formed in FP64 and the rational in
+// FP32 (u > 100 in region I, so no cancellation: ~1e-7 relative), in FP64 where |x| or y
+// reaches 1e9 (there u^2 could leave the FP32 range).
 __device__ __forceinline__ double region1_line(double nu, double g, double y, float amp, double f) {
   const double x = (nu_if(nu) - f) * g;
+  if (fabs(x) < 1e9 && y < 1e9) {
+    const float xf = (float)x, yf = (float)y;
+    const float u = __builtin_fmaf(xf, xf, __builtin_fmaf(yf, yf, -0.5f));
+    const float q = (u + 1.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, 2.0f * (yf * yf)));
+    return (double)((amp * (0.5641896f * yf)) * q);
+  }
   const double y2 = y * y;
   const double u = x * x + y2 - 0.5;
   return (double)amp * 0.5641896 * y * (u + 1.0) / (u * u + 2.0 * y2);
