@@ -944,18 +944,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (lane == 0 && pairs) atomicAdd(P.far_evals, pairs);
 }
 
-// Region I of one line at bin f, scaled by amp = S g: c y (u + 1) / (u^2 + 2 y^2) with
-// u = x^2 + y^2 - 1/2 (the region-I kernel's form).  x is formed in FP64 and the rational in
-// FP32 (u > 100 in region I, so no cancellation: ~1e-7 relative), in FP64 where |x| or y
-// reaches 1e9 (there u^2 could leave the FP32 range).
+// Region I of lines [a, z) at bin f in FP64, line by line (the same form as the region-I
+// kernel's: c y (u + 1) / (u^2 + 2 y^2), u = x^2 + y^2 - 1/2, scaled by amp = S g).
 __device__ __forceinline__ double region1_line(double nu, double g, double y, float amp, double f) {
   const double x = (nu_if(nu) - f) * g;
-  if (fabs(x) < 1e9 && y < 1e9) {
-    const float xf = (float)x, yf = (float)y;
-    const float u = __builtin_fmaf(xf, xf, __builtin_fmaf(yf, yf, -0.5f));
-    const float q = (u + 1.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, 2.0f * (yf * yf)));
-    return (double)((amp * (0.5641896f * yf)) * q);
-  }
   const double y2 = y * y;
   const double u = x * x + y2 - 0.5;
   return (double)amp * 0.5641896 * y * (u + 1.0) / (u * u + 2.0 * y2);

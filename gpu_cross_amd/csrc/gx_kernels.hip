@@ -315,12 +315,24 @@ __device__ __forceinline__ void prep_mixed_store_t(const PrepT& t, const MixedLi
 // One thread per line over all npts points: the line's record is read once, and the
 // temperature-dependent part (two exp, one pow) is computed and stored once per run of
 // consecutive points at one temperature (row tq counts the runs from 0, as
-// mixed_temperature_rows counts them on the host).
+// mixed_temperature_rows counts them on the host).  Which points start a run is decided once
+// per wave, a lane per point, into wave masks (points beyond kPrepMaskPts compare their
+// constants in the loop).
+constexpr int kPrepMasks = 4;
+constexpr int kPrepMaskPts = 64 * kPrepMasks;
+
 __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restrict__ pcs, int npts, LinesView lv,
                                                     GridView gv, const int* __restrict__ IB,
                                                     const int* __restrict__ IE, MixedLineData out,
                                                     unsigned long long* __restrict__ core_evals) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  unsigned long long starts[kPrepMasks];
+#pragma unroll
+  for (int c = 0; c < kPrepMasks; ++c) {
+    const int q = 64 * c + lane;
+    starts[c] = __ballot(q > 0 && q < npts && !same_t(pcs[q], pcs[q - 1]));
+  }
   unsigned long long w = 0;
   if (i < lv.n) {
     const double nu = lv.nu[i];
@@ -332,7 +344,9 @@ __global__ __launch_bounds__(256) void k_prep_mixed(const PointConsts* __restric
     int tq = 0;
     prep_mixed_store_t(t, out, i, 0);
     for (int p = 0; p < npts; ++p) {
-      if (p > 0 && !same_t(pcs[p], pcs[p - 1])) {
+      const bool run = p < kPrepMaskPts ? ((starts[p >> 6] >> (p & 63)) & 1ull) != 0
+                                        : !same_t(pcs[p], pcs[p - 1]);
+      if (run) {
         t = prep_mixed_t(pcs[p], nu_if, a, gi, elow, nexp);
         prep_mixed_store_t(t, out, i, ++tq);
       }
