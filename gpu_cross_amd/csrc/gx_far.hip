@@ -1208,12 +1208,16 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
 
 // Window-end lines, one workgroup per (task, block), one bin per thread: bin j gets the
 // lines ending inside the block with ie > j and those starting inside it with ib <= j,
-// each range as at most one partial leaf line by line plus whole leaves (leaves_sum).
-template <int R, bool CPLX>
-__global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
+// each range as at most one partial leaf line by line plus whole leaves (leaves_sum).  Two
+// kernels share the per-bin split: k_far_edge_parts (PARTS) adds the partial leaves (gathered
+// line loads in few registers, so that many waves hide their latency; every task, and it
+// counts the pairs), k_far_edges the whole leaves' series of the tasks of one moment kind
+// (block scans, many registers).
+template <int R, bool CPLX, bool PARTS>
+__device__ __forceinline__ void far_edges(const MixedPlan& P) {
   constexpr int NB = 64 * R;
-  __shared__ int s_cut[NB];
-  __shared__ double s_tot[R * RangeMoments<CPLX>::N];
+  __shared__ int s_cut[PARTS ? 1 : NB];
+  __shared__ double s_tot[PARTS ? 1 : R * RangeMoments<CPLX>::N];
   __shared__ double s_red[R];
   // (block, point) pairs in block-major order, dealt to the XCDs (workgroups go to them in
   // turn) in contiguous runs: an XCD takes the points of one block one after another, so the
@@ -1227,13 +1231,13 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   const size_t slot = (size_t)task * P.nblocks + b;
   const int4 er = P.edge_rng[slot];
   if (er.x >= er.y && er.z >= er.w) return;                    // set for whole blocks only
-  if ((P.far_kind[task] != 0) != CPLX) return;                 // the other kind's launch
+  if (!PARTS && (P.far_kind[task] != 0) != CPLX) return;       // the other kind's launch
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
   const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const int w0 = gv.start_idx + b * NB;
   const int j = w0 + threadIdx.x;
-  const double f = gv.freq[j], f_lo = gv.freq[w0], f_hi = gv.freq[w0 + NB - 1];
+  const double f = gv.freq[j];
   unsigned long long pairs = 0;
   double acc = 0.0;
   // The partial leaf of a range at bin j: its lines in the range one by one, or, when they are
@@ -1243,35 +1247,51 @@ __global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   if (er.x < er.y) {                                           // er.y is a leaf boundary
     const int s0 = first_above<false>(P, er.x, er.y, j);       // ie > j from here
     int A = (s0 + kFarLeaf - 1) / kFarLeaf;
-    pairs += (unsigned long long)(er.y - s0);
-    const int part = A * kFarLeaf - s0;
-    if (2 * part > kFarLeaf) {
-      acc -= region1_lines(P, tk, (A - 1) * kFarLeaf, s0, f);
-      --A;
+    const bool cpl = 2 * (A * kFarLeaf - s0) > kFarLeaf;
+    if (PARTS) {
+      pairs += (unsigned long long)(er.y - s0);
+      if (cpl) acc -= region1_lines(P, tk, (A - 1) * kFarLeaf, s0, f);
+      else acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
     } else {
-      acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
+      if (cpl) --A;
+      acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, gv.freq[w0], gv.freq[w0 + NB - 1], s_cut,
+                                       s_tot, s_red, task, j);
     }
-    acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   if (er.z < er.w) {                                           // er.z is a leaf boundary
     const int p1 = first_above<true>(P, er.z, er.w, j);        // ib <= j before here
     int D = p1 / kFarLeaf;
-    pairs += (unsigned long long)(p1 - er.z);
     const int leaf_end = min((D + 1) * kFarLeaf, P.nlines);
-    const int part = p1 - D * kFarLeaf;
-    if (2 * part > leaf_end - D * kFarLeaf) {
-      acc -= region1_lines(P, tk, p1, leaf_end, f);
-      ++D;
+    const bool cpl = 2 * (p1 - D * kFarLeaf) > leaf_end - D * kFarLeaf;
+    if (PARTS) {
+      pairs += (unsigned long long)(p1 - er.z);
+      if (cpl) acc -= region1_lines(P, tk, p1, leaf_end, f);
+      else acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
     } else {
-      acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
+      if (cpl) ++D;
+      acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, gv.freq[w0], gv.freq[w0 + NB - 1], s_cut,
+                                        s_tot, s_red, task, j);
     }
-    acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   tk.out[j] += acc;
-  // pairs of this wave, counted once
+  if (PARTS) {
+    // pairs of this wave, counted once
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
-  if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(P.far_evals, pairs);
+    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+    if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(P.far_evals, pairs);
+  }
+}
+
+template <int R, bool CPLX>
+__global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
+  far_edges<R, CPLX, false>(P);
+}
+
+// ~118 VGPRs, four waves per SIMD (capped at 80 the eight lines' loads spill, 19.8 ms against
+// 16.0 ms per 48-point table).
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_far_edge_parts(MixedPlan P) {
+  far_edges<R, false, true>(P);
 }
 
 }  // namespace
@@ -1308,6 +1328,7 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
       k_far_eval<1, false><<<grid, 256, 0, s>>>(plan);
       k_far_eval<1, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<1><<<near_grid<1>(plan), 256, 0, s>>>(plan);
+      k_far_edge_parts<1><<<grid_blocks, 64, 0, s>>>(plan);
       k_far_edges<1, false><<<grid_blocks, 64, 0, s>>>(plan);
       k_far_edges<1, true><<<grid_blocks, 64, 0, s>>>(plan);
       break;
@@ -1315,6 +1336,7 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
       k_far_eval<2, false><<<grid, 256, 0, s>>>(plan);
       k_far_eval<2, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<2><<<near_grid<2>(plan), 256, 0, s>>>(plan);
+      k_far_edge_parts<2><<<grid_blocks, 128, 0, s>>>(plan);
       k_far_edges<2, false><<<grid_blocks, 128, 0, s>>>(plan);
       k_far_edges<2, true><<<grid_blocks, 128, 0, s>>>(plan);
       break;
@@ -1322,6 +1344,7 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
       k_far_eval<4, false><<<grid, 256, 0, s>>>(plan);
       k_far_eval<4, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<4><<<near_grid<4>(plan), 256, 0, s>>>(plan);
+      k_far_edge_parts<4><<<grid_blocks, 256, 0, s>>>(plan);
       k_far_edges<4, false><<<grid_blocks, 256, 0, s>>>(plan);
       k_far_edges<4, true><<<grid_blocks, 256, 0, s>>>(plan);
       break;
@@ -1329,6 +1352,7 @@ hipError_t launch_far_eval(const MixedPlan& plan, int R, hipStream_t s) {
       k_far_eval<8, false><<<grid, 256, 0, s>>>(plan);
       k_far_eval<8, true><<<grid, 256, 0, s>>>(plan);
       if (plan.near) k_near_eval<8><<<near_grid<8>(plan), 256, 0, s>>>(plan);
+      k_far_edge_parts<8><<<grid_blocks, 512, 0, s>>>(plan);
       k_far_edges<8, false><<<grid_blocks, 512, 0, s>>>(plan);
       k_far_edges<8, true><<<grid_blocks, 512, 0, s>>>(plan);
       break;
