@@ -1208,16 +1208,14 @@ __device__ __forceinline__ double leaves_sum(const MixedPlan& P, const MixedTask
 
 // Window-end lines, one workgroup per (task, block), one bin per thread: bin j gets the
 // lines ending inside the block with ie > j and those starting inside it with ib <= j,
-// each range as at most one partial leaf line by line plus whole leaves (leaves_sum).  Two
-// kernels share the per-bin split: k_far_edge_parts (PARTS) adds the partial leaves (gathered
-// line loads in few registers, so that many waves hide their latency; every task, and it
-// counts the pairs), k_far_edges the whole leaves' series of the tasks of one moment kind
-// (block scans, many registers).
-template <int R, bool CPLX, bool PARTS>
-__device__ __forceinline__ void far_edges(const MixedPlan& P) {
+// each range as at most one partial leaf line by line plus whole leaves (leaves_sum).  The
+// whole leaves' series are added here, for the tasks of one moment kind (block scans, many
+// registers); the partial leaves by k_far_edge_parts, with the same split.
+template <int R, bool CPLX>
+__global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
   constexpr int NB = 64 * R;
-  __shared__ int s_cut[PARTS ? 1 : NB];
-  __shared__ double s_tot[PARTS ? 1 : R * RangeMoments<CPLX>::N];
+  __shared__ int s_cut[NB];
+  __shared__ double s_tot[R * RangeMoments<CPLX>::N];
   __shared__ double s_red[R];
   // (block, point) pairs in block-major order, dealt to the XCDs (workgroups go to them in
   // turn) in contiguous runs: an XCD takes the points of one block one after another, so the
@@ -1231,14 +1229,13 @@ __device__ __forceinline__ void far_edges(const MixedPlan& P) {
   const size_t slot = (size_t)task * P.nblocks + b;
   const int4 er = P.edge_rng[slot];
   if (er.x >= er.y && er.z >= er.w) return;                    // set for whole blocks only
-  if (!PARTS && (P.far_kind[task] != 0) != CPLX) return;       // the other kind's launch
+  if ((P.far_kind[task] != 0) != CPLX) return;                 // the other kind's launch
   const MixedTask tk = P.tasks[task];
   const GridView gv = P.gv;
   const FarVal* recs = P.far_rec + (size_t)task * P.far_nclus * kFarRec;
   const int w0 = gv.start_idx + b * NB;
   const int j = w0 + threadIdx.x;
-  const double f = gv.freq[j];
-  unsigned long long pairs = 0;
+  const double f = gv.freq[j], f_lo = gv.freq[w0], f_hi = gv.freq[w0 + NB - 1];
   double acc = 0.0;
   // The partial leaf of a range at bin j: its lines in the range one by one, or, when they are
   // more than half of it, the leaf's series (it joins the whole leaves; every leaf the range
@@ -1247,51 +1244,130 @@ __device__ __forceinline__ void far_edges(const MixedPlan& P) {
   if (er.x < er.y) {                                           // er.y is a leaf boundary
     const int s0 = first_above<false>(P, er.x, er.y, j);       // ie > j from here
     int A = (s0 + kFarLeaf - 1) / kFarLeaf;
-    const bool cpl = 2 * (A * kFarLeaf - s0) > kFarLeaf;
-    if (PARTS) {
-      pairs += (unsigned long long)(er.y - s0);
-      if (cpl) acc -= region1_lines(P, tk, (A - 1) * kFarLeaf, s0, f);
-      else acc += region1_lines(P, tk, s0, A * kFarLeaf, f);
-    } else {
-      if (cpl) --A;
-      acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, gv.freq[w0], gv.freq[w0 + NB - 1], s_cut,
-                                       s_tot, s_red, task, j);
-    }
+    if (2 * (A * kFarLeaf - s0) > kFarLeaf) --A;
+    acc += leaves_sum<R, CPLX, true>(P, tk, recs, A, er.y / kFarLeaf, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   if (er.z < er.w) {                                           // er.z is a leaf boundary
     const int p1 = first_above<true>(P, er.z, er.w, j);        // ib <= j before here
     int D = p1 / kFarLeaf;
     const int leaf_end = min((D + 1) * kFarLeaf, P.nlines);
-    const bool cpl = 2 * (p1 - D * kFarLeaf) > leaf_end - D * kFarLeaf;
-    if (PARTS) {
-      pairs += (unsigned long long)(p1 - er.z);
-      if (cpl) acc -= region1_lines(P, tk, p1, leaf_end, f);
-      else acc += region1_lines(P, tk, D * kFarLeaf, p1, f);
-    } else {
-      if (cpl) ++D;
-      acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, gv.freq[w0], gv.freq[w0 + NB - 1], s_cut,
-                                        s_tot, s_red, task, j);
-    }
+    if (2 * (p1 - D * kFarLeaf) > leaf_end - D * kFarLeaf) ++D;
+    acc += leaves_sum<R, CPLX, false>(P, tk, recs, er.z / kFarLeaf, D, f, f_lo, f_hi, s_cut, s_tot, s_red, task, j);
   }
   tk.out[j] += acc;
-  if (PARTS) {
-    // pairs of this wave, counted once
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
-    if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(P.far_evals, pairs);
+}
+
+// A lane's double from lane i of the wave (wave-uniform result).
+__device__ __forceinline__ double readlane_f64(double v, int i) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {   // every source lane valid for these controls
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sums over each half (32 lanes) of the wave, fixed order: quad swaps by 1 and 2, row rotations
+// by 4 and 8 (every lane of a row of 16 then holds a row total), then rows 0 + 1 and 2 + 3 by the
+// 16-lane swap.  Lanes 0 and 32 hold the halves' totals.  Every lane of the wave must be active.
+__device__ __forceinline__ double half_sums(double v) {
+#pragma clang fp contract(off)
+  v = v + dpp_f64<0xB1>(v);               // quad_perm [1, 0, 3, 2]
+  v = v + dpp_f64<0x4E>(v);               // quad_perm [2, 3, 0, 1]
+  v = v + dpp_f64<0x124>(v);              // row_ror 4
+  v = v + dpp_f64<0x128>(v);              // row_ror 8
+  float alo = __int_as_float(__double2loint(v)), ahi = __int_as_float(__double2hiint(v));
+  float blo = alo, bhi = ahi;
+  lane_swap<16>(alo, blo);
+  lane_swap<16>(ahi, bhi);
+  return __hiloint2double(__float_as_int(ahi), __float_as_int(alo)) +
+         __hiloint2double(__float_as_int(bhi), __float_as_int(blo));
+}
+
+// Region I of one line at bin f, scaled by amp = S g: c y (u + 1) / (u^2 + 2 y^2) with
+// u = x^2 + y^2 - 1/2 (the region-I kernel's form).  x is formed in FP64 and the rational in
+// FP32 (u > 100 in region I, so no cancellation: ~1e-7 relative), in FP64 where |x| or y
+// reaches 1e9 (there u^2 could leave the FP32 range).
+__device__ __forceinline__ double region1_line_mixed(double nu, double g, double y, float amp, double f) {
+  const double x = (nu_if(nu) - f) * g;
+  if (fabs(x) < 1e9 && y < 1e9) {
+    const float xf = (float)x, yf = (float)y;
+    const float u = __builtin_fmaf(xf, xf, __builtin_fmaf(yf, yf, -0.5f));
+    const float q = (u + 1.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, 2.0f * (yf * yf)));
+    return (double)((amp * (0.5641896f * yf)) * q);
   }
+  return region1_line(nu, g, y, amp, f);
 }
 
-template <int R, bool CPLX>
-__global__ __launch_bounds__(64 * R) void k_far_edges(MixedPlan P) {
-  far_edges<R, CPLX, false>(P);
-}
-
-// ~118 VGPRs, four waves per SIMD (capped at 80 the eight lines' loads spill, 19.8 ms against
-// 16.0 ms per 48-point table).
+// Partial leaves of the window-end ranges.  Bin j's part of the range ending inside the block
+// is the line set [a0, z0) of one leaf, of the range starting inside it [a1, z1), each at most
+// 32 lines (the complement form keeps it so) and added or, for the complement, subtracted (the
+// split of far_edges).  The wave takes its bins one by one: lanes 0-31 evaluate the first set
+// of bin i, lanes 32-63 the second, with coalesced loads of consecutive lines; the two halves
+// are summed over their lanes in a fixed order and added to bin i's thread.  (One bin per
+// thread walking its own lines made every load a 64-line gather: 16 ms per 48-point table,
+// bound by address processing.)
 template <int R>
 __global__ __launch_bounds__(64 * R) void k_far_edge_parts(MixedPlan P) {
-  far_edges<R, false, true>(P);
+  constexpr int NB = 64 * R;
+  const int per = gridDim.x / 8;                               // as far_edges maps them
+  const int pair = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (pair >= P.nblocks * P.ntasks) return;
+  const int b = pair / P.ntasks;
+  const int task = pair - b * P.ntasks;
+  const int4 er = P.edge_rng[(size_t)task * P.nblocks + b];
+  if (er.x >= er.y && er.z >= er.w) return;                    // set for whole blocks only
+  const MixedTask tk = P.tasks[task];
+  const int lane = threadIdx.x & 63;
+  const int j = P.gv.start_idx + b * NB + threadIdx.x;
+  int a0 = 0, z0 = 0, a1 = 0, z1 = 0;
+  bool n0 = false, n1 = false;
+  unsigned long long pairs = 0;
+  if (er.x < er.y) {                                           // er.y is a leaf boundary
+    const int s0 = first_above<false>(P, er.x, er.y, j);       // ie > j from here
+    const int A = (s0 + kFarLeaf - 1) / kFarLeaf;
+    pairs += (unsigned long long)(er.y - s0);
+    n0 = 2 * (A * kFarLeaf - s0) > kFarLeaf;
+    a0 = n0 ? (A - 1) * kFarLeaf : s0;
+    z0 = n0 ? s0 : A * kFarLeaf;
+  }
+  if (er.z < er.w) {                                           // er.z is a leaf boundary
+    const int p1 = first_above<true>(P, er.z, er.w, j);        // ib <= j before here
+    const int D = p1 / kFarLeaf;
+    const int leaf_end = min((D + 1) * kFarLeaf, P.nlines);
+    pairs += (unsigned long long)(p1 - er.z);
+    n1 = 2 * (p1 - D * kFarLeaf) > leaf_end - D * kFarLeaf;
+    a1 = n1 ? p1 : D * kFarLeaf;
+    z1 = n1 ? leaf_end : p1;
+  }
+  const double f = P.gv.freq[j];
+  const bool second = lane >= 32;
+  const int hl = lane & 31;
+  double acc = 0.0;
+  for (int i = 0; i < 64; ++i) {
+    const int sa0 = __builtin_amdgcn_readlane(a0, i), sz0 = __builtin_amdgcn_readlane(z0, i);
+    const int sa1 = __builtin_amdgcn_readlane(a1, i), sz1 = __builtin_amdgcn_readlane(z1, i);
+    if (sa0 >= sz0 && sa1 >= sz1) continue;                    // wave-uniform
+    const int l = (second ? sa1 : sa0) + hl;
+    double t = 0.0;
+    if (l < (second ? sz1 : sz0)) {
+      const double fi = readlane_f64(f, i);
+      const double g = tk.g[l];
+      t = region1_line_mixed(P.nu[l], g, mixed_y(tk, l, g), tk.amp[l], fi);
+    }
+    t = half_sums(t);
+    const double t0 = readlane_f64(t, 0), t1 = readlane_f64(t, 32);
+    if (lane == i) acc = (n0 ? -t0 : t0) + (n1 ? -t1 : t1);    // each bin's sets once
+  }
+  P.tasks[task].out[j] += acc;
+  // pairs of this wave, counted once
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+  if (lane == 0 && pairs) atomicAdd(P.far_evals, pairs);
 }
 
 }  // namespace
