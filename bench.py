@@ -33,8 +33,8 @@ FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compe
 # table's autotuner picks R): HBM bytes per launch of the region-I kernel (2 FETCH_SIZE +
 # WRITE_SIZE, copy-calibrated), and the step's VALU issue use, sum over every kernel of one
 # 48-point table of SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x clock x step time).
-PROFILED_TRAFFIC = {("table_48", 1): 1.725e11, "ingest_1e8": None}
-PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.872, "step_ms": 394.8, "valu_insts": 2.036e11, "clock_ghz": 2.31}}
+PROFILED_TRAFFIC = {("table_48", 1): 1.724e11, "ingest_1e8": None}
+PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.877, "step_ms": 384.7, "valu_insts": 1.993e11, "clock_ghz": 2.31}}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -216,7 +216,7 @@ def bench_table(args, wl):
         "core_pass": {"kernel": "k_core_mixed (W4 regions II-IV; region I of near ranges)", "ms": core_ms, "evals": core_evals,
                       "evals_per_s": core_evals / (core_ms * 1e-3) if core_ms > 0 else None},
         "far_field": {"kernels": "k_far_kind, k_far_geom, k_far_leaf, k_far_up, k_far_plan, k_edge_plan, k_far_eval, "
-                                 "k_near_eval, k_far_edges (FP32 leaf sums and records, FP64 tree shifts and series; near leaves summed "
+                                 "k_near_eval, k_far_edge_parts, k_far_edges (FP32 leaf sums and records, FP64 tree shifts and series; near leaves summed "
                                  "per bin)",
                       "ms": far_ms, "evals": far_evals, "share": far_evals / max(evals, 1)},
         "intensity_kernels_ms": eager_int,
