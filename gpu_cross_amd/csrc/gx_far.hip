@@ -956,8 +956,9 @@ __device__ __forceinline__ double region1_line(double nu, double g, double y, fl
 __device__ __forceinline__ double region1_lines(const MixedPlan& P, const MixedTask& tk, int a, int z, double f) {
   double s = 0.0;
   int l = a;
-  // eight lines' loads issued together (the loop is bound by their latency: four per batch
-  // made the window-end kernels ~9 ms slower per table), then their terms added in line order
+  // eight lines' loads issued together (a thread walking its own lines is bound by their
+  // latency), then their terms added in line order; the fallback of leaves_sum when a range's
+  // series misses its bound
   for (; l + 8 <= z; l += 8) {
     double nu[8], g[8], y[8];
     float amp[8];
