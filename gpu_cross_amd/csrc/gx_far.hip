@@ -417,6 +417,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
   }
   r = (double)f32_bound<true>(r);                // the stored FP32 value, rounded up
   if (!(r > 0.0)) r = 1.0;
+  const double ir = rcp_nr(r);
   if (P.near) {                                  // hull of the children's near ranges
     int2 u = make_int2(0, 0);
     for (int h = 0; h < nch; ++h) {
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     for (int h = 0; h < nch; ++h) {
       const int ch = P.far_off[lev - 1] + 2 * i + h;
       const FarRecC cr{tb + ch, ns};
-      const double a = (P.far_geom[ch].x - c) / r, b = cr[0] / r;
+      const double a = (P.far_geom[ch].x - c) * ir, b = cr[0] * ir;
       // Q_k (k = 1..P) += sum_{m=1..k} C(k, m) a^(k-m) b^m Q_m: v_m = b^m Q_m (v_0 = Q_0 = 0 for a
       // real centre), then P in-place Taylor-shift passes v_k += a v_(k-1)
       double v[kFarP + 1];
@@ -467,7 +468,7 @@ __global__ __launch_bounds__(256) void k_far_up(MixedPlan P, int lev) {
     for (int h = 0; h < nch; ++h) {
       const int ch = P.far_off[lev - 1] + 2 * i + h;
       const FarRecC cr{tb + ch, ns};
-      const double ar = (P.far_geom[ch].x - c) / r, ai = (cr[5] - gc) / r, b = cr[0] / r;
+      const double ar = (P.far_geom[ch].x - c) * ir, ai = (cr[5] - gc) * ir, b = cr[0] * ir;
       double vr[kFarP], vi[kFarP];
       double bm = 1.0;
 #pragma unroll
@@ -1040,9 +1041,10 @@ struct RangeMoments {
 template <bool CPLX>
 __device__ __forceinline__ void leaf_to_range(FarRecC rc, double nu_l, double cE, double gE,
                                               double rE, double (&m)[RangeMoments<CPLX>::N]) {
-  const double b = rc[0] / rE;
+  const double irE = rcp_nr(rE);
+  const double b = rc[0] * irE;
   if (!CPLX) {
-    const double a = (nu_l - cE) / rE;
+    const double a = (nu_l - cE) * irE;
     double v[kRangeP + 1];
     v[0] = 0.0;                                                // Q_0 of a real centre is 0
     double bm = b;
@@ -1058,7 +1060,7 @@ __device__ __forceinline__ void leaf_to_range(FarRecC rc, double nu_l, double cE
 #pragma unroll
     for (int k = 1; k <= kRangeP; ++k) m[k - 1] += v[k];
   } else {
-    const double ar = (nu_l - cE) / rE, ai = (rc[5] - gE) / rE;
+    const double ar = (nu_l - cE) * irE, ai = (rc[5] - gE) * irE;
     double vr[kRangeP], vi[kRangeP];
     double bm = 1.0;
 #pragma unroll
