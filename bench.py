@@ -33,8 +33,8 @@ FLOP_PER_EVAL = 20          # region-I W4 closed form + offset + scaling + compe
 # table's autotuner picks R): HBM bytes per launch of the region-I kernel (2 FETCH_SIZE +
 # WRITE_SIZE, copy-calibrated), and the step's VALU issue use, sum over every kernel of one
 # 48-point table of SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x clock x step time).
-PROFILED_TRAFFIC = {("table_48", 1): 1.724e11, "ingest_1e8": None}
-PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.877, "step_ms": 384.7, "valu_insts": 1.993e11, "clock_ghz": 2.31}}
+PROFILED_TRAFFIC = {("table_48", 1): 1.725e11, "ingest_1e8": None}
+PROFILED_STEP_VALU = {("table_48", 1): {"value": 0.874, "step_ms": 381.7, "valu_insts": 1.970e11, "clock_ghz": 2.31}}
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
